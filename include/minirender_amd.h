@@ -1,0 +1,173 @@
+/*
+ * minirender_amd.h -- C-ABI of the MI355X (gfx950) rasterization path.
+ *
+ * This is the boundary under minirender::Renderer::render(): plain C types,
+ * plain pointers and sizes, an int status on every call (0 = ok, negative =
+ * error), nothing thrown across it.  The reference is a C++ class with no FFI
+ * (src/Renderer.cpp:311-381, include/minirender/Renderer.h:17-64); each entry
+ * point below names the reference code whose work it takes over.
+ *
+ * Frame state that the reference derives on the host inside render()
+ * (scene flattening Scene.cpp:13-36, modelview/normal matrix
+ * Renderer.cpp:336-338, light to eye space :317-320) is computed by the
+ * caller and passed in mr_frame / mr_draw, so the host math is shared by every
+ * consumer of a frame.  The near-plane depth (:324-326) is derived here from
+ * the projection, in fp32, as the reference does.
+ */
+#ifndef MINIRENDER_AMD_H
+#define MINIRENDER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define MR_OK 0
+#define MR_ERR_ARG -1      /* bad argument (null pointer, size out of range)   */
+#define MR_ERR_HIP -2      /* a HIP runtime call failed                         */
+#define MR_ERR_NOMEM -3    /* device allocation failed                          */
+#define MR_ERR_BUDGET -4   /* request does not fit the context's HBM budget    */
+#define MR_ERR_HANDLE -5   /* unknown mesh / texture / ticket handle            */
+#define MR_ERR_NOGPU -6    /* no HIP device available                           */
+#define MR_ERR_INDEX -7    /* mesh index out of range of its attribute array   */
+#define MR_ERR_DEVICE -8   /* a kernel reported an internal error (spin limit) */
+#define MR_ERR_TIMEOUT -9  /* a wait timed out                                   */
+#define MR_ERR_CLOSED -10  /* server is shutting down                            */
+
+/* ---- texture formats ----------------------------------------------------- */
+#define MR_TEX_RGB8 0    /* 3 bytes per texel, texel value = byte / 255.0f (io.cpp:405-411) */
+#define MR_TEX_RGB32F 1  /* 3 floats per texel, as Material::texture (Scene.h:42)         */
+
+/* ---- one draw = one Renderable (Scene.h:47-53) with its material ------- */
+typedef struct mr_draw {
+  int32_t mesh;        /* handle from mr_mesh_upload                             */
+  int32_t texture;     /* handle from mr_texture_upload, or -1 for none          */
+  float modelview[16]; /* row-major, view * node transform (Renderer.cpp:337)     */
+  float normalmat[16]; /* row-major, inverse(modelview) transposed (:338)         */
+  float diffuse[3];    /* Material fields (Scene.h:38-45)                          */
+  float specular[3];
+  float emissive[3];
+  float shininess;
+} mr_draw;
+
+/* ---- one frame = one Renderer::render() call --------------------------- */
+typedef struct mr_frame {
+  int32_t width, height;  /* Renderer::setSize (Renderer.cpp:100-106)           */
+  float projection[16];   /* row-major, Renderer::setProjection                  */
+  float background[3];    /* Renderer::setBackground, clear() (:113-119)         */
+  float light[3];         /* eye-space light: view*light if point, else light   */
+                          /* normalized (Renderer.cpp:317-320)                   */
+  int32_t light_is_point; /* setLight(v, point)                                  */
+  float ambient;          /* Scene::ambientLight (:322)                           */
+  int32_t lighting;       /* setLighting                                          */
+  int32_t texturing;      /* setTexturing                                         */
+  int32_t save_normals;   /* setSaveNormals                                       */
+  int32_t ndraws;
+  const mr_draw* draws;   /* in Scene::collectShapes order (draw order)          */
+} mr_frame;
+
+/* Outputs of a frame.  Any pointer may be NULL (not produced).  With
+ * on_device = 0 they are host pointers and the call copies results back;
+ * with on_device = 1 they are device pointers on the context's device. */
+typedef struct mr_outputs {
+  uint8_t* rgb8;     /* H*W*3 RGB8, quantized as savePPM (io.cpp:358-361)         */
+  float* depth;      /* H*W fp32 z-buffer, 1e11f where nothing was drawn (:116)  */
+  uint32_t* winner;  /* H*W draw-order index of the visible triangle:              */
+                     /* 2*(triangles of earlier draws + triangle) + clip sub-index */
+                     /* (0/1); 0xFFFFFFFF where nothing was drawn                 */
+  float* rgb;        /* H*W*3 unquantized colour, as Renderer::getImage (:383)    */
+  float* normals;    /* H*W*3 eye-space normals, as getNormalsImage (h:63)        */
+  int32_t on_device;
+} mr_outputs;
+
+/* Per-frame statistics of the last call on a context. */
+typedef struct mr_stats {
+  int64_t triangles_in;       /* input triangles                               */
+  int64_t triangles_clipped;  /* input triangles that crossed the near plane   */
+  int64_t records;            /* triangles after clip/reject/cull (raster set) */
+  int64_t bin_entries;        /* (tile, triangle) pairs                          */
+  int64_t tiles;              /* screen tiles                                     */
+  int64_t retries;            /* re-runs after a bin-capacity overflow           */
+  double stage_ms[8];         /* per-stage GPU time when profiling is on:        */
+                              /* 0 clear, 1 geometry, 2 scan, 3 bin, 4 raster     */
+  double frame_ms;            /* first to last kernel of the frame (profiling)  */
+} mr_stats;
+
+/* ---- device context: HIP device, stream, resident meshes/textures ------- */
+typedef struct mr_ctx mr_ctx;
+
+/* Number of HIP devices visible (0 without a GPU). */
+int mr_device_count(int32_t* count);
+
+/* Create a context on `device`.  hbm_budget bytes (0 = no limit) caps the
+ * scratch and framebuffer memory the context may hold at once. */
+int mr_ctx_create(int32_t device, uint64_t hbm_budget, mr_ctx** out);
+int mr_ctx_destroy(mr_ctx* ctx);
+
+/* Use an external HIP stream (hipStream_t) for all work of this context;
+ * NULL restores the context's own stream. */
+int mr_ctx_set_stream(mr_ctx* ctx, void* stream);
+int mr_ctx_get_stream(mr_ctx* ctx, void** stream);
+
+/* Tile configuration: tile_w x tile_h pixels per raster workgroup
+ * (each of 8, 16, 32).  Defaults to 16 x 16. */
+int mr_ctx_set_tile(mr_ctx* ctx, int32_t tile_w, int32_t tile_h);
+/* Bin-list capacity in (tile, triangle) entries per input triangle
+ * (>= 1); overflow re-runs the frame with a larger list. */
+int mr_ctx_set_bin_factor(mr_ctx* ctx, float entries_per_triangle);
+/* Record per-stage HIP event times into mr_stats (adds event overhead). */
+int mr_ctx_set_profiling(mr_ctx* ctx, int32_t on);
+int mr_ctx_get_stats(mr_ctx* ctx, mr_stats* stats);
+/* Bytes of device memory currently held by the context. */
+int mr_ctx_memory(mr_ctx* ctx, uint64_t* resident_bytes, uint64_t* scratch_bytes);
+
+/* Upload a TriMesh (Scene.h:73-87): vertices, normals, texcoords and the three
+ * index arrays (3 per triangle).  normal_indices may equal indices; uvs /
+ * uv_indices may be NULL (then uv = (0,0) as in Renderer.cpp:371-379).
+ * Indices are checked against the attribute counts (MR_ERR_INDEX).
+ * Replaces the per-mesh copies of Renderer::paintMesh (:340-349). */
+int mr_mesh_upload(mr_ctx* ctx, const float* positions, int32_t npositions,
+                   const float* normals, int32_t nnormals, const float* uvs,
+                   int32_t nuvs, const int32_t* indices,
+                   const int32_t* normal_indices, const int32_t* uv_indices,
+                   int32_t ntriangles, int32_t* handle);
+int mr_mesh_release(mr_ctx* ctx, int32_t handle);
+
+/* Upload a texture (Material::texture, Scene.h:42), rows x cols texels. */
+int mr_texture_upload(mr_ctx* ctx, const void* texels, int32_t format,
+                      int32_t rows, int32_t cols, int32_t* handle);
+int mr_texture_release(mr_ctx* ctx, int32_t handle);
+
+/* Render one frame: Renderer::render() (Renderer.cpp:311-332) for the draws
+ * of `frame`.  Synchronous: returns when outputs are written. */
+int mr_render(mr_ctx* ctx, const mr_frame* frame, const mr_outputs* out);
+
+/* Render n independent frames in one batched pass (all stages launched once
+ * for the whole batch).  Equivalent to n calls of mr_render. */
+int mr_render_batch(mr_ctx* ctx, const mr_frame* frames, int32_t nframes,
+                    const mr_outputs* outs);
+
+/* ---- host helpers of the reference API (no GPU needed) ----------------- */
+/* Projection matrices, row-major, Renderer.cpp:40-83. */
+int mr_projection_ortho(float l, float r, float b, float t, float n, float f,
+                        float out[16]);
+int mr_projection_perspective(float l, float r, float b, float t, float n,
+                              float f, float out[16]);
+int mr_projection_frustum(float fovy, float aspect, float n, float f,
+                          float out[16]);
+int mr_projection_frustum_h(float fovx, float aspect, float n, float f,
+                            float out[16]);
+int mr_projection_ortho_fov(float fov, float aspect, float n, float f,
+                            float out[16]);
+int mr_projection_cv(const float K[16], float w, float h, float n, float f,
+                     float out[16]);
+/* Near-plane eye depth derived from a projection (Renderer.cpp:324-326). */
+int mr_znear(const float projection[16], float* znear);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MINIRENDER_AMD_H */

@@ -1,0 +1,199 @@
+"""minirender_amd -- minirender's rasterization path on MI355X (gfx950).
+
+Python host side of the C-ABI in include/minirender_amd.h.  `Context` keeps
+meshes and textures resident on the GPU and renders FrameSpecs
+(minirender_amd.scenes) with the HIP pipeline.  The HIP library must be built
+(`python -m minirender_amd.build`); nothing here falls back to the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Iterable, List, Optional, Sequence
+
+import numpy as np
+
+from . import scenes
+from ._capi import (MR_TEX_RGB8, MR_TEX_RGB32F, NO_WINNER, MrError, check, load_library,
+                    mr_draw, mr_frame, mr_outputs, mr_stats)
+from .scenes import Draw, FrameSpec, Material, Mesh, Texture
+
+__all__ = ["Context", "FrameSpec", "Draw", "Material", "Mesh", "Texture", "MrError",
+           "NO_WINNER", "scenes", "frame_struct", "load_library", "OUTPUT_KINDS"]
+
+OUTPUT_KINDS = ("rgb8", "depth", "winner", "rgb", "normals")
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f(seq, n):
+    return (C.c_float * n)(*[float(v) for v in np.asarray(seq, np.float32).ravel()[:n]])
+
+
+def frame_struct(frame: FrameSpec, mesh_handle, tex_handle):
+    """mr_frame for `frame`; draws map meshes/textures through the handle
+    callbacks.  Returns (mr_frame, keepalive)."""
+    n = len(frame.draws)
+    draws = (mr_draw * max(n, 1))()
+    for i, d in enumerate(frame.draws):
+        mv = scenes.modelview(frame, d)
+        nm = scenes.normal_matrix(mv)
+        s = draws[i]
+        s.mesh = mesh_handle(d.mesh)
+        s.texture = tex_handle(d.material.texture) if d.material.texture is not None else -1
+        s.modelview = _f(mv, 16)
+        s.normalmat = _f(nm, 16)
+        s.diffuse = _f(d.material.diffuse, 3)
+        s.specular = _f(d.material.specular, 3)
+        s.emissive = _f(d.material.emissive, 3)
+        s.shininess = float(np.float32(d.material.shininess))
+    f = mr_frame()
+    f.width, f.height = int(frame.width), int(frame.height)
+    f.projection = _f(frame.projection, 16)
+    f.background = _f(frame.background, 3)
+    f.light = _f(scenes.eye_light(frame), 3)
+    f.light_is_point = int(bool(frame.light_is_point))
+    f.ambient = float(np.float32(frame.ambient))
+    f.lighting = int(bool(frame.lighting))
+    f.texturing = int(bool(frame.texturing))
+    f.save_normals = int(bool(frame.save_normals))
+    f.ndraws = n
+    f.draws = C.cast(draws, C.POINTER(mr_draw))
+    return f, draws
+
+
+def alloc_outputs(frame: FrameSpec, kinds: Iterable[str]) -> Dict[str, np.ndarray]:
+    h, w = frame.height, frame.width
+    shapes = {"rgb8": ((h, w, 3), np.uint8), "depth": ((h, w), np.float32),
+              "winner": ((h, w), np.uint32), "rgb": ((h, w, 3), np.float32),
+              "normals": ((h, w, 3), np.float32)}
+    return {k: np.empty(*shapes[k]) for k in kinds}
+
+
+def outputs_struct(out: Dict[str, np.ndarray]) -> mr_outputs:
+    o = mr_outputs()
+    for k in OUTPUT_KINDS:
+        setattr(o, k, _ptr(out.get(k)))
+    o.on_device = 0
+    return o
+
+
+class Context:
+    """A device context with a resident mesh/texture pool."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        check(self.lib.mr_ctx_create(device, hbm_budget, C.byref(h)), "mr_ctx_create")
+        self.h = h
+        self._meshes: Dict[int, tuple] = {}
+        self._textures: Dict[int, tuple] = {}
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.mr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- configuration -------------------------------------------------------
+    def set_tile(self, tw: int, th: int) -> None:
+        check(self.lib.mr_ctx_set_tile(self.h, tw, th), "mr_ctx_set_tile")
+
+    def set_bin_factor(self, f: float) -> None:
+        check(self.lib.mr_ctx_set_bin_factor(self.h, f), "mr_ctx_set_bin_factor")
+
+    def set_profiling(self, on: bool) -> None:
+        check(self.lib.mr_ctx_set_profiling(self.h, int(on)), "mr_ctx_set_profiling")
+
+    def set_stream(self, stream: int) -> None:
+        check(self.lib.mr_ctx_set_stream(self.h, C.c_void_p(stream)), "mr_ctx_set_stream")
+
+    def stats(self) -> mr_stats:
+        s = mr_stats()
+        check(self.lib.mr_ctx_get_stats(self.h, C.byref(s)), "mr_ctx_get_stats")
+        return s
+
+    def memory(self):
+        r, s = C.c_uint64(), C.c_uint64()
+        check(self.lib.mr_ctx_memory(self.h, C.byref(r), C.byref(s)), "mr_ctx_memory")
+        return r.value, s.value
+
+    # -- resident pool -------------------------------------------------------
+    def mesh_handle(self, m: Mesh) -> int:
+        ent = self._meshes.get(id(m))
+        if ent is not None and ent[1] is m:
+            return ent[0]
+        h = C.c_int32()
+        uvs = m.uvs if m.has_uv else None
+        tix = m.uv_indices if m.has_uv else None
+        nix = m.nidx
+        check(self.lib.mr_mesh_upload(
+            self.h, _ptr(m.positions), len(m.positions), _ptr(m.normals), len(m.normals),
+            _ptr(uvs), 0 if uvs is None else len(uvs), _ptr(m.indices), _ptr(nix), _ptr(tix),
+            m.ntri, C.byref(h)), "mr_mesh_upload")
+        self._meshes[id(m)] = (h.value, m)
+        return h.value
+
+    def release_mesh(self, m: Mesh) -> None:
+        ent = self._meshes.pop(id(m), None)
+        if ent is not None:
+            check(self.lib.mr_mesh_release(self.h, ent[0]), "mr_mesh_release")
+
+    def texture_handle(self, t: Texture) -> int:
+        ent = self._textures.get(id(t))
+        if ent is not None and ent[1] is t:
+            return ent[0]
+        tex = np.ascontiguousarray(t.texels)
+        fmt = MR_TEX_RGB8 if tex.dtype == np.uint8 else MR_TEX_RGB32F
+        if fmt == MR_TEX_RGB32F:
+            tex = np.ascontiguousarray(tex, dtype=np.float32)
+        h = C.c_int32()
+        check(self.lib.mr_texture_upload(self.h, _ptr(tex), fmt, t.rows, t.cols, C.byref(h)),
+              "mr_texture_upload")
+        self._textures[id(t)] = (h.value, t)
+        return h.value
+
+    def frame(self, frame: FrameSpec):
+        return frame_struct(frame, self.mesh_handle, self.texture_handle)
+
+    # -- rendering -----------------------------------------------------------
+    def render(self, frame: FrameSpec, kinds: Sequence[str] = ("rgb8", "depth", "winner")):
+        f, keep = self.frame(frame)
+        out = alloc_outputs(frame, kinds)
+        o = outputs_struct(out)
+        check(self.lib.mr_render(self.h, C.byref(f), C.byref(o)), "mr_render")
+        return out
+
+    def render_device(self, frame: FrameSpec, ptrs: Dict[str, int]) -> None:
+        """Render into device buffers (e.g. torch tensors' data_ptr())."""
+        f, keep = self.frame(frame)
+        o = mr_outputs()
+        for k in OUTPUT_KINDS:
+            setattr(o, k, C.c_void_p(ptrs[k]) if ptrs.get(k) else None)
+        o.on_device = 1
+        check(self.lib.mr_render(self.h, C.byref(f), C.byref(o)), "mr_render")
+
+    def render_batch(self, frames: List[FrameSpec],
+                     kinds: Sequence[str] = ("rgb8", "depth", "winner")):
+        structs, keep, outs = [], [], []
+        for fr in frames:
+            f, k = self.frame(fr)
+            structs.append(f)
+            keep.append(k)
+            outs.append(alloc_outputs(fr, kinds))
+        farr = (mr_frame * len(frames))(*structs)
+        oarr = (mr_outputs * len(frames))(*[outputs_struct(o) for o in outs])
+        check(self.lib.mr_render_batch(self.h, farr, len(frames), oarr), "mr_render_batch")
+        return outs

@@ -1,0 +1,137 @@
+"""ctypes mirror of include/minirender_amd.h and the library loader.
+
+The HIP library is loaded from this package directory only; a missing library
+raises -- there is no CPU fallback in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libminirender_amd.so"
+
+MR_OK = 0
+ERRORS = {
+    -1: "MR_ERR_ARG", -2: "MR_ERR_HIP", -3: "MR_ERR_NOMEM", -4: "MR_ERR_BUDGET",
+    -5: "MR_ERR_HANDLE", -6: "MR_ERR_NOGPU", -7: "MR_ERR_INDEX", -8: "MR_ERR_DEVICE",
+    -9: "MR_ERR_TIMEOUT", -10: "MR_ERR_CLOSED",
+}
+MR_TEX_RGB8 = 0
+MR_TEX_RGB32F = 1
+NO_WINNER = 0xFFFFFFFF
+
+
+class MrError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what} failed: {ERRORS.get(code, code)}")
+        self.code = code
+
+
+class mr_draw(C.Structure):
+    _fields_ = [
+        ("mesh", C.c_int32), ("texture", C.c_int32),
+        ("modelview", C.c_float * 16), ("normalmat", C.c_float * 16),
+        ("diffuse", C.c_float * 3), ("specular", C.c_float * 3),
+        ("emissive", C.c_float * 3), ("shininess", C.c_float),
+    ]
+
+
+class mr_frame(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32),
+        ("projection", C.c_float * 16), ("background", C.c_float * 3),
+        ("light", C.c_float * 3), ("light_is_point", C.c_int32),
+        ("ambient", C.c_float), ("lighting", C.c_int32),
+        ("texturing", C.c_int32), ("save_normals", C.c_int32),
+        ("ndraws", C.c_int32), ("draws", C.POINTER(mr_draw)),
+    ]
+
+
+class mr_outputs(C.Structure):
+    _fields_ = [
+        ("rgb8", C.c_void_p), ("depth", C.c_void_p), ("winner", C.c_void_p),
+        ("rgb", C.c_void_p), ("normals", C.c_void_p), ("on_device", C.c_int32),
+    ]
+
+
+class mr_stats(C.Structure):
+    _fields_ = [
+        ("triangles_in", C.c_int64), ("triangles_clipped", C.c_int64),
+        ("records", C.c_int64), ("bin_entries", C.c_int64), ("tiles", C.c_int64),
+        ("retries", C.c_int64), ("stage_ms", C.c_double * 8), ("frame_ms", C.c_double),
+    ]
+
+
+class mr_server_config(C.Structure):
+    _fields_ = [
+        ("max_batch", C.c_int32), ("max_inflight", C.c_int32),
+        ("hbm_budget", C.c_uint64), ("use_graphs", C.c_int32),
+        ("batch_window_us", C.c_int32),
+    ]
+
+
+class mr_request_stats(C.Structure):
+    _fields_ = [
+        ("arrival_us", C.c_double), ("admit_us", C.c_double),
+        ("done_us", C.c_double), ("batch_id", C.c_int64), ("batch_size", C.c_int32),
+        ("status", C.c_int32),
+    ]
+
+
+_P = C.c_void_p
+_I = C.c_int32
+_F = C.c_float
+_FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int32)
+
+# name -> (restype, argtypes).  Every symbol the header declares.
+SIGNATURES = {
+    "mr_device_count": (C.c_int, [_IP]),
+    "mr_ctx_create": (C.c_int, [_I, C.c_uint64, C.POINTER(_P)]),
+    "mr_ctx_destroy": (C.c_int, [_P]),
+    "mr_ctx_set_stream": (C.c_int, [_P, _P]),
+    "mr_ctx_get_stream": (C.c_int, [_P, C.POINTER(_P)]),
+    "mr_ctx_set_tile": (C.c_int, [_P, _I, _I]),
+    "mr_ctx_set_bin_factor": (C.c_int, [_P, _F]),
+    "mr_ctx_set_profiling": (C.c_int, [_P, _I]),
+    "mr_ctx_get_stats": (C.c_int, [_P, C.POINTER(mr_stats)]),
+    "mr_ctx_memory": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "mr_mesh_upload": (C.c_int, [_P, _P, _I, _P, _I, _P, _I, _P, _P, _P, _I, _IP]),
+    "mr_mesh_release": (C.c_int, [_P, _I]),
+    "mr_texture_upload": (C.c_int, [_P, _P, _I, _I, _I, _IP]),
+    "mr_texture_release": (C.c_int, [_P, _I]),
+    "mr_render": (C.c_int, [_P, C.POINTER(mr_frame), C.POINTER(mr_outputs)]),
+    "mr_render_batch": (C.c_int, [_P, C.POINTER(mr_frame), _I, C.POINTER(mr_outputs)]),
+    "mr_projection_ortho": (C.c_int, [_F, _F, _F, _F, _F, _F, _FP]),
+    "mr_projection_perspective": (C.c_int, [_F, _F, _F, _F, _F, _F, _FP]),
+    "mr_projection_frustum": (C.c_int, [_F, _F, _F, _F, _FP]),
+    "mr_projection_frustum_h": (C.c_int, [_F, _F, _F, _F, _FP]),
+    "mr_projection_ortho_fov": (C.c_int, [_F, _F, _F, _F, _FP]),
+    "mr_projection_cv": (C.c_int, [_FP, _F, _F, _F, _F, _FP]),
+    "mr_znear": (C.c_int, [_FP, _FP]),
+}
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Load the in-tree HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with `python -m minirender_amd.build`"
+            )
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != MR_OK:
+        raise MrError(rc, what)

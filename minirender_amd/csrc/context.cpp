@@ -1,0 +1,570 @@
+// context.cpp -- C-ABI (include/minirender_amd.h) over the HIP pipeline.
+//
+// A context owns one device, one stream, the resident meshes and textures
+// (Renderer::paintMesh's per-frame copies, Renderer.cpp:340-349, become one
+// upload) and the scratch of the batch in flight.  mr_render is
+// Renderer::render() (Renderer.cpp:311-332) for one frame; mr_render_batch
+// runs many frames through each kernel once.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "minirender_amd.h"
+#include "common.h"
+#include "exact.h"
+#include "pipeline.h"
+#include "context.h"
+
+using namespace mr;
+
+namespace mr {
+
+int dev_alloc(mr_ctx* c, DevBuf& b, size_t bytes, bool scratch) {
+  if (bytes == 0) bytes = 16;
+  if (b.p && b.bytes >= bytes) return MR_OK;
+  size_t want = b.p ? std::max(bytes, b.bytes + b.bytes / 2) : bytes;
+  want = (want + 255) & ~(size_t)255;
+  uint64_t& acct = scratch ? c->scratch_bytes : c->resident_bytes;
+  if (scratch && c->budget && acct - b.bytes + want > c->budget) {
+    if (acct - b.bytes + bytes > c->budget) return MR_ERR_BUDGET;
+    want = bytes;
+  }
+  if (b.p) {
+    hipFree(b.p);
+    acct -= b.bytes;
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  if (hipMalloc(&b.p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    return MR_ERR_NOMEM;
+  }
+  b.bytes = want;
+  acct += want;
+  return MR_OK;
+}
+
+void dev_free(mr_ctx* c, DevBuf& b, bool scratch) {
+  if (b.p) {
+    hipFree(b.p);
+    (scratch ? c->scratch_bytes : c->resident_bytes) -= b.bytes;
+  }
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+int host_alloc(HostBuf& b, size_t bytes) {
+  if (b.p && b.bytes >= bytes) return MR_OK;
+  if (b.p) hipHostFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  size_t want = std::max<size_t>(bytes, 4096);
+  if (hipHostMalloc(&b.p, want, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    return MR_ERR_NOMEM;
+  }
+  b.bytes = want;
+  return MR_OK;
+}
+
+void host_free(HostBuf& b) {
+  if (b.p) hipHostFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+#define HIP_TRY(x)                               \
+  do {                                           \
+    hipError_t e_ = (x);                         \
+    if (e_ != hipSuccess) {                      \
+      (void)hipGetLastError();                   \
+      return MR_ERR_HIP;                         \
+    }                                            \
+  } while (0)
+
+// Describe frames as jobs + draws; sizes the batch.
+int build_batch(mr_ctx* c, const mr_frame* frames, int nframes,
+                BatchPlan& plan) {
+  plan.jobs.assign(nframes, JobDesc{});
+  plan.draws.clear();
+  plan.total_tris = 0;
+  plan.ntiles = 0;
+  const int tw = c->tile_w, th = c->tile_h;
+  for (int j = 0; j < nframes; j++) {
+    const mr_frame* f = &frames[j];
+    if (f->width < 1 || f->height < 1 || f->width > kMaxDim ||
+        f->height > kMaxDim || f->ndraws < 0 || (f->ndraws > 0 && !f->draws))
+      return MR_ERR_ARG;
+    JobDesc& J = plan.jobs[j];
+    memcpy(J.P, f->projection, sizeof(J.P));
+    J.znear = znear_of(f->projection);
+    J.persp = f->projection[15] == 0.0f;
+    J.W = f->width;
+    J.H = f->height;
+    J.tiles_x = (f->width + tw - 1) / tw;
+    J.tiles_y = (f->height + th - 1) / th;
+    J.tile_base = (int32_t)plan.ntiles;
+    if ((int64_t)plan.ntiles + (int64_t)J.tiles_x * J.tiles_y > 0x7FFFFFFF)
+      return MR_ERR_ARG;
+    plan.ntiles += J.tiles_x * J.tiles_y;
+    J.draw_begin = (int32_t)plan.draws.size();
+    for (int i = 0; i < 3; i++) {
+      J.bg[i] = f->background[i];
+      J.light[i] = f->light[i];
+    }
+    J.light_is_point = f->light_is_point != 0;
+    J.ambient = f->ambient;
+    J.lighting = f->lighting != 0;
+    J.save_normals = f->save_normals != 0;
+    uint64_t order = 0;
+    for (int d = 0; d < f->ndraws; d++) {
+      const mr_draw& s = f->draws[d];
+      if (s.mesh < 0 || s.mesh >= (int)c->meshes.size() || !c->meshes[s.mesh].alive)
+        return MR_ERR_HANDLE;
+      if (s.texture >= 0 && (s.texture >= (int)c->textures.size() ||
+                             !c->textures[s.texture].alive))
+        return MR_ERR_HANDLE;
+      const MeshRec& m = c->meshes[s.mesh];
+      DrawDesc D;
+      memset(&D, 0, sizeof(D));
+      memcpy(D.mv, s.modelview, sizeof(D.mv));
+      memcpy(D.nm, s.normalmat, sizeof(D.nm));
+      for (int i = 0; i < 3; i++) {
+        D.diffuse[i] = s.diffuse[i];
+        D.specular[i] = s.specular[i];
+        D.emissive[i] = s.emissive[i];
+      }
+      D.shininess = s.shininess;
+      D.pos = (const float*)m.pos.p;
+      D.nrm = (const float*)m.nrm.p;
+      D.idx = (const int32_t*)m.idx.p;
+      D.nidx = (const int32_t*)(m.nidx_alias ? m.idx.p : m.nidx.p);
+      if (m.has_uv) {
+        D.uv = (const float*)m.uv.p;
+        D.tidx = (const int32_t*)(m.tidx_alias ? m.idx.p : m.tidx.p);
+      }
+      if (f->texturing && s.texture >= 0) {
+        const TexRec& t = c->textures[s.texture];
+        if (t.rows > 0 && t.cols > 0) {
+          D.tex = t.texels.p;
+          D.tex_format = t.fmt;
+          D.tex_rows = t.rows;
+          D.tex_cols = t.cols;
+        }
+      }
+      D.ntri = m.ntri;
+      D.job = j;
+      D.tri_begin = plan.total_tris;
+      D.order_base = (uint32_t)order;
+      order += 2ull * (uint64_t)m.ntri;
+      plan.total_tris += m.ntri;
+      plan.draws.push_back(D);
+    }
+    if (order >= 0xFFFFFFFFull) return MR_ERR_ARG;
+    J.draw_end = (int32_t)plan.draws.size();
+  }
+  if (2 * plan.total_tris >= 0xFFFFFFFFll) return MR_ERR_ARG;
+  return MR_OK;
+}
+
+static int ensure_scratch(mr_ctx* c, const BatchPlan& p, uint64_t bins_cap) {
+  Scratch& s = c->sc;
+  const int64_t T = p.total_tris;
+  const int gch = geom_chunks_for(T), scb = scan_blocks_for(p.ntiles);
+  int rc;
+  if ((rc = dev_alloc(c, s.recs, (size_t)(2 * T) * sizeof(TriRec), true))) return rc;
+  if ((rc = dev_alloc(c, s.tile_count, (size_t)p.ntiles * 4, true))) return rc;
+  if ((rc = dev_alloc(c, s.tile_off, (size_t)p.ntiles * 4, true))) return rc;
+  if ((rc = dev_alloc(c, s.tile_cursor, (size_t)p.ntiles * 4, true))) return rc;
+  if ((rc = dev_alloc(c, s.bins, (size_t)bins_cap * 4, true))) return rc;
+  if ((rc = dev_alloc(c, s.geom_status, (size_t)std::max(gch, 1) * 8, true))) return rc;
+  if ((rc = dev_alloc(c, s.scan_status, (size_t)std::max(scb, 1) * 8, true))) return rc;
+  if ((rc = dev_alloc(c, s.counters, kNumCounters * 8, true))) return rc;
+  if ((rc = dev_alloc(c, s.draws, std::max<size_t>(p.draws.size(), 1) * sizeof(DrawDesc), true))) return rc;
+  if ((rc = dev_alloc(c, s.jobs, p.jobs.size() * sizeof(JobDesc), true))) return rc;
+  if ((rc = host_alloc(s.h_draws, std::max<size_t>(p.draws.size(), 1) * sizeof(DrawDesc)))) return rc;
+  if ((rc = host_alloc(s.h_jobs, p.jobs.size() * sizeof(JobDesc)))) return rc;
+  if ((rc = host_alloc(s.h_counters, kNumCounters * 8))) return rc;
+  return MR_OK;
+}
+
+BatchArgs make_args(mr_ctx* c, const BatchPlan& p) {
+  Scratch& s = c->sc;
+  BatchArgs a;
+  memset(&a, 0, sizeof(a));
+  a.recs = (TriRec*)s.recs.p;
+  a.tile_count = (uint32_t*)s.tile_count.p;
+  a.tile_off = (uint32_t*)s.tile_off.p;
+  a.tile_cursor = (uint32_t*)s.tile_cursor.p;
+  a.bins = (uint32_t*)s.bins.p;
+  a.bins_cap = s.bins.bytes / 4;
+  if (a.bins_cap > 0xFFFFFFF0ull) a.bins_cap = 0xFFFFFFF0ull;
+  a.geom_status = (unsigned long long*)s.geom_status.p;
+  a.scan_status = (unsigned long long*)s.scan_status.p;
+  a.counters = (unsigned long long*)s.counters.p;
+  a.draws = (const DrawDesc*)s.draws.p;
+  a.ndraws = (int32_t)p.draws.size();
+  a.jobs = (const JobDesc*)s.jobs.p;
+  a.njobs = (int32_t)p.jobs.size();
+  a.total_tris = p.total_tris;
+  a.ntiles = (int32_t)p.ntiles;
+  a.tile_w = c->tile_w;
+  a.tile_h = c->tile_h;
+  a.geom_chunks = geom_chunks_for(p.total_tris);
+  a.scan_blocks = scan_blocks_for((int32_t)p.ntiles);
+  a.num_cus = c->num_cus;
+  return a;
+}
+
+// Launch all stages of a batch on the context stream (no host sync).
+int launch_batch(mr_ctx* c, const BatchArgs& a, bool profile) {
+  hipStream_t st = c->stream;
+  if (profile) HIP_TRY(hipEventRecord(c->ev[0], st));
+  HIP_TRY(launch_clear(a, st));
+  if (profile) HIP_TRY(hipEventRecord(c->ev[1], st));
+  HIP_TRY(launch_geometry(a, st));
+  if (profile) HIP_TRY(hipEventRecord(c->ev[2], st));
+  HIP_TRY(launch_scan(a, st));
+  if (profile) HIP_TRY(hipEventRecord(c->ev[3], st));
+  HIP_TRY(launch_bin(a, st));
+  if (profile) HIP_TRY(hipEventRecord(c->ev[4], st));
+  HIP_TRY(launch_tile(a, st));
+  if (profile) HIP_TRY(hipEventRecord(c->ev[5], st));
+  return MR_OK;
+}
+
+// Runs a batch whose jobs' output pointers are device pointers.  Handles a
+// bin-list overflow by growing the list and re-running.  Synchronous.
+int run_batch(mr_ctx* c, BatchPlan& p) {
+  uint64_t want_bins = std::max<uint64_t>(
+      (uint64_t)((double)p.total_tris * c->bin_factor) + (uint64_t)p.ntiles, 1u << 16);
+  if (c->sc.bins.bytes / 4 > want_bins) want_bins = c->sc.bins.bytes / 4;
+  int rc = ensure_scratch(c, p, want_bins);
+  if (rc) return rc;
+  Scratch& s = c->sc;
+  memcpy(s.h_jobs.p, p.jobs.data(), p.jobs.size() * sizeof(JobDesc));
+  if (!p.draws.empty()) memcpy(s.h_draws.p, p.draws.data(), p.draws.size() * sizeof(DrawDesc));
+  HIP_TRY(hipMemcpyAsync(s.jobs.p, s.h_jobs.p, p.jobs.size() * sizeof(JobDesc),
+                         hipMemcpyHostToDevice, c->stream));
+  if (!p.draws.empty())
+    HIP_TRY(hipMemcpyAsync(s.draws.p, s.h_draws.p, p.draws.size() * sizeof(DrawDesc),
+                           hipMemcpyHostToDevice, c->stream));
+  unsigned long long* hc = (unsigned long long*)s.h_counters.p;
+  c->stats.retries = 0;
+  for (int attempt = 0;; attempt++) {
+    BatchArgs a = make_args(c, p);
+    if ((rc = launch_batch(c, a, c->profiling))) return rc;
+    HIP_TRY(hipMemcpyAsync(hc, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (hc[kError]) return MR_ERR_DEVICE;
+    if (!hc[kBinOverflow]) break;
+    if (attempt >= 3 || hc[kBinEntries] >= 0xFFFFFFF0ull) return MR_ERR_NOMEM;
+    c->stats.retries++;
+    const uint64_t need = hc[kBinEntries] + hc[kBinEntries] / 4 + 1024;
+    if ((rc = dev_alloc(c, s.bins, (size_t)need * 4, true))) return rc;
+  }
+  c->stats.triangles_in = p.total_tris;
+  c->stats.triangles_clipped = (int64_t)hc[kClipped];
+  c->stats.records = (int64_t)hc[kRecords];
+  c->stats.bin_entries = (int64_t)hc[kBinEntries];
+  c->stats.tiles = p.ntiles;
+  if (c->profiling) {
+    float ms = 0;
+    for (int i = 0; i < 5; i++) {
+      hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]);
+      c->stats.stage_ms[i] = ms;
+    }
+    hipEventElapsedTime(&ms, c->ev[0], c->ev[5]);
+    c->stats.frame_ms = ms;
+  }
+  return MR_OK;
+}
+
+// Frames with host outputs: render into device staging, copy back.
+int render_frames(mr_ctx* c, const mr_frame* frames, int n, const mr_outputs* outs) {
+  BatchPlan p;
+  int rc = build_batch(c, frames, n, p);
+  if (rc) return rc;
+  // Device staging for host outputs.
+  size_t total = 0;
+  std::vector<size_t> offs(n * 5, 0);
+  for (int j = 0; j < n; j++) {
+    const mr_outputs& o = outs[j];
+    const size_t npix = (size_t)frames[j].width * frames[j].height;
+    const size_t sz[5] = {npix * 3, npix * 4, npix * 4, npix * 12, npix * 12};
+    void* ptrs[5] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals};
+    for (int k = 0; k < 5; k++) {
+      offs[j * 5 + k] = total;
+      if (ptrs[k] && !o.on_device) total += (sz[k] + 255) & ~(size_t)255;
+    }
+  }
+  if (total && (rc = dev_alloc(c, c->sc.outputs, total, true))) return rc;
+  char* base = (char*)c->sc.outputs.p;
+  for (int j = 0; j < n; j++) {
+    const mr_outputs& o = outs[j];
+    JobDesc& J = p.jobs[j];
+    if (o.on_device) {
+      J.rgb8 = o.rgb8; J.depth = o.depth; J.winner = o.winner; J.rgb = o.rgb; J.normals = o.normals;
+    } else {
+      J.rgb8 = o.rgb8 ? (uint8_t*)(base + offs[j * 5 + 0]) : nullptr;
+      J.depth = o.depth ? (float*)(base + offs[j * 5 + 1]) : nullptr;
+      J.winner = o.winner ? (uint32_t*)(base + offs[j * 5 + 2]) : nullptr;
+      J.rgb = o.rgb ? (float*)(base + offs[j * 5 + 3]) : nullptr;
+      J.normals = o.normals ? (float*)(base + offs[j * 5 + 4]) : nullptr;
+    }
+  }
+  if ((rc = run_batch(c, p))) return rc;
+  bool copied = false;
+  for (int j = 0; j < n; j++) {
+    const mr_outputs& o = outs[j];
+    if (o.on_device) continue;
+    const JobDesc& J = p.jobs[j];
+    const size_t npix = (size_t)J.W * J.H;
+    if (o.rgb8) { HIP_TRY(hipMemcpyAsync(o.rgb8, J.rgb8, npix * 3, hipMemcpyDeviceToHost, c->stream)); copied = true; }
+    if (o.depth) { HIP_TRY(hipMemcpyAsync(o.depth, J.depth, npix * 4, hipMemcpyDeviceToHost, c->stream)); copied = true; }
+    if (o.winner) { HIP_TRY(hipMemcpyAsync(o.winner, J.winner, npix * 4, hipMemcpyDeviceToHost, c->stream)); copied = true; }
+    if (o.rgb) { HIP_TRY(hipMemcpyAsync(o.rgb, J.rgb, npix * 12, hipMemcpyDeviceToHost, c->stream)); copied = true; }
+    if (o.normals) { HIP_TRY(hipMemcpyAsync(o.normals, J.normals, npix * 12, hipMemcpyDeviceToHost, c->stream)); copied = true; }
+  }
+  if (copied) HIP_TRY(hipStreamSynchronize(c->stream));
+  return MR_OK;
+}
+
+}  // namespace mr
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int mr_device_count(int32_t* count) {
+  if (!count) return MR_ERR_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    n = 0;
+  }
+  *count = n;
+  return MR_OK;
+}
+
+int mr_ctx_create(int32_t device, uint64_t hbm_budget, mr_ctx** out) {
+  if (!out) return MR_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return MR_ERR_NOGPU;
+  }
+  if (device < 0 || device >= n) return MR_ERR_ARG;
+  HIP_TRY(hipSetDevice(device));
+  mr_ctx* c = new (std::nothrow) mr_ctx();
+  if (!c) return MR_ERR_NOMEM;
+  c->device = device;
+  c->budget = hbm_budget;
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return MR_ERR_HIP;
+  }
+  c->stream = c->own_stream;
+  for (auto& e : c->ev) hipEventCreate(&e);
+  int cus = 0;
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  c->num_cus = cus > 0 ? cus : 256;
+  *out = c;
+  return MR_OK;
+}
+
+int mr_ctx_destroy(mr_ctx* c) {
+  if (!c) return MR_ERR_ARG;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  for (auto& m : c->meshes) {
+    dev_free(c, m.pos, false); dev_free(c, m.nrm, false); dev_free(c, m.uv, false);
+    dev_free(c, m.idx, false); dev_free(c, m.nidx, false); dev_free(c, m.tidx, false);
+  }
+  for (auto& t : c->textures) dev_free(c, t.texels, false);
+  Scratch& s = c->sc;
+  DevBuf* bufs[] = {&s.recs, &s.tile_count, &s.tile_off, &s.tile_cursor, &s.bins,
+                    &s.geom_status, &s.scan_status, &s.counters, &s.draws, &s.jobs, &s.outputs};
+  for (DevBuf* b : bufs) dev_free(c, *b, true);
+  host_free(s.h_draws);
+  host_free(s.h_jobs);
+  host_free(s.h_counters);
+  for (auto& e : c->ev) hipEventDestroy(e);
+  hipStreamDestroy(c->own_stream);
+  delete c;
+  return MR_OK;
+}
+
+int mr_ctx_set_stream(mr_ctx* c, void* stream) {
+  if (!c) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  return MR_OK;
+}
+
+int mr_ctx_get_stream(mr_ctx* c, void** stream) {
+  if (!c || !stream) return MR_ERR_ARG;
+  *stream = (void*)c->stream;
+  return MR_OK;
+}
+
+int mr_ctx_set_tile(mr_ctx* c, int32_t tw, int32_t th) {
+  if (!c || !tile_supported(tw, th)) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  c->tile_w = tw;
+  c->tile_h = th;
+  return MR_OK;
+}
+
+int mr_ctx_set_bin_factor(mr_ctx* c, float f) {
+  if (!c || !(f >= 1.0f) || f > 1024.0f) return MR_ERR_ARG;
+  c->bin_factor = f;
+  return MR_OK;
+}
+
+int mr_ctx_set_profiling(mr_ctx* c, int32_t on) {
+  if (!c) return MR_ERR_ARG;
+  c->profiling = on != 0;
+  return MR_OK;
+}
+
+int mr_ctx_get_stats(mr_ctx* c, mr_stats* st) {
+  if (!c || !st) return MR_ERR_ARG;
+  *st = c->stats;
+  return MR_OK;
+}
+
+int mr_ctx_memory(mr_ctx* c, uint64_t* resident, uint64_t* scratch) {
+  if (!c) return MR_ERR_ARG;
+  if (resident) *resident = c->resident_bytes;
+  if (scratch) *scratch = c->scratch_bytes;
+  return MR_OK;
+}
+
+static int upload(mr_ctx* c, DevBuf& b, const void* src, size_t bytes) {
+  int rc = dev_alloc(c, b, bytes, false);
+  if (rc) return rc;
+  if (bytes) HIP_TRY(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+  return MR_OK;
+}
+
+static bool indices_ok(const int32_t* ix, int64_t n, int32_t limit) {
+  for (int64_t i = 0; i < n; i++)
+    if (ix[i] < 0 || ix[i] >= limit) return false;
+  return true;
+}
+
+int mr_mesh_upload(mr_ctx* c, const float* positions, int32_t npos,
+                   const float* normals, int32_t nnrm, const float* uvs,
+                   int32_t nuv, const int32_t* indices,
+                   const int32_t* normal_indices, const int32_t* uv_indices,
+                   int32_t ntri, int32_t* handle) {
+  if (!c || !handle || npos < 0 || nnrm < 0 || nuv < 0 || ntri < 0) return MR_ERR_ARG;
+  if (ntri > 0 && (!positions || !normals || !indices || !normal_indices)) return MR_ERR_ARG;
+  if (ntri >= (1 << 30)) return MR_ERR_ARG;
+  const bool has_uv = uvs && uv_indices && nuv > 0;
+  const int64_t ni = 3 * (int64_t)ntri;
+  if (!indices_ok(indices, ni, npos) || !indices_ok(normal_indices, ni, nnrm) ||
+      (has_uv && !indices_ok(uv_indices, ni, nuv)))
+    return MR_ERR_INDEX;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  MeshRec m;
+  m.npos = npos; m.nnrm = nnrm; m.nuv = has_uv ? nuv : 0; m.ntri = ntri;
+  m.has_uv = has_uv;
+  m.nidx_alias = normal_indices == indices;
+  m.tidx_alias = has_uv && uv_indices == indices;
+  int rc;
+  if ((rc = upload(c, m.pos, positions, (size_t)npos * 12)) ||
+      (rc = upload(c, m.nrm, normals, (size_t)nnrm * 12)) ||
+      (rc = upload(c, m.idx, indices, (size_t)ni * 4)) ||
+      (!m.nidx_alias && (rc = upload(c, m.nidx, normal_indices, (size_t)ni * 4))) ||
+      (has_uv && (rc = upload(c, m.uv, uvs, (size_t)nuv * 8))) ||
+      (has_uv && !m.tidx_alias && (rc = upload(c, m.tidx, uv_indices, (size_t)ni * 4)))) {
+    dev_free(c, m.pos, false); dev_free(c, m.nrm, false); dev_free(c, m.uv, false);
+    dev_free(c, m.idx, false); dev_free(c, m.nidx, false); dev_free(c, m.tidx, false);
+    return rc;
+  }
+  m.alive = true;
+  for (size_t i = 0; i < c->meshes.size(); i++)
+    if (!c->meshes[i].alive) {
+      c->meshes[i] = m;
+      *handle = (int32_t)i;
+      return MR_OK;
+    }
+  c->meshes.push_back(m);
+  *handle = (int32_t)c->meshes.size() - 1;
+  return MR_OK;
+}
+
+int mr_mesh_release(mr_ctx* c, int32_t h) {
+  if (!c) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (h < 0 || h >= (int)c->meshes.size() || !c->meshes[h].alive) return MR_ERR_HANDLE;
+  hipStreamSynchronize(c->stream);
+  MeshRec& m = c->meshes[h];
+  dev_free(c, m.pos, false); dev_free(c, m.nrm, false); dev_free(c, m.uv, false);
+  dev_free(c, m.idx, false); dev_free(c, m.nidx, false); dev_free(c, m.tidx, false);
+  m.alive = false;
+  return MR_OK;
+}
+
+int mr_texture_upload(mr_ctx* c, const void* texels, int32_t format,
+                      int32_t rows, int32_t cols, int32_t* handle) {
+  if (!c || !handle || rows < 0 || cols < 0 || (rows * (int64_t)cols > 0 && !texels))
+    return MR_ERR_ARG;
+  if (format != MR_TEX_RGB8 && format != MR_TEX_RGB32F) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  TexRec t;
+  t.fmt = format; t.rows = rows; t.cols = cols;
+  const size_t bytes = (size_t)rows * cols * (format == MR_TEX_RGB8 ? 3 : 12);
+  int rc = upload(c, t.texels, texels, bytes);
+  if (rc) return rc;
+  t.alive = true;
+  for (size_t i = 0; i < c->textures.size(); i++)
+    if (!c->textures[i].alive) {
+      c->textures[i] = t;
+      *handle = (int32_t)i;
+      return MR_OK;
+    }
+  c->textures.push_back(t);
+  *handle = (int32_t)c->textures.size() - 1;
+  return MR_OK;
+}
+
+int mr_texture_release(mr_ctx* c, int32_t h) {
+  if (!c) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (h < 0 || h >= (int)c->textures.size() || !c->textures[h].alive) return MR_ERR_HANDLE;
+  hipStreamSynchronize(c->stream);
+  dev_free(c, c->textures[h].texels, false);
+  c->textures[h].alive = false;
+  return MR_OK;
+}
+
+int mr_render(mr_ctx* c, const mr_frame* frame, const mr_outputs* out) {
+  if (!c || !frame || !out) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  return render_frames(c, frame, 1, out);
+}
+
+int mr_render_batch(mr_ctx* c, const mr_frame* frames, int32_t n, const mr_outputs* outs) {
+  if (!c || !frames || !outs || n < 1) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  return render_frames(c, frames, n, outs);
+}
+
+}  // extern "C"

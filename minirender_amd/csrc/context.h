@@ -1,0 +1,87 @@
+// context.h -- host-side state behind the C-ABI (context.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "minirender_amd.h"
+#include "common.h"
+#include "pipeline.h"
+
+namespace mr {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct HostBuf {  // pinned (hipHostMalloc)
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+// A resident TriMesh (Scene.h:73-87).
+struct MeshRec {
+  DevBuf pos, nrm, uv, idx, nidx, tidx;
+  int32_t npos = 0, nnrm = 0, nuv = 0, ntri = 0;
+  bool has_uv = false;
+  bool nidx_alias = false;  // normal indices == position indices
+  bool tidx_alias = false;  // uv indices == position indices
+  bool alive = false;
+};
+
+struct TexRec {
+  DevBuf texels;
+  int32_t fmt = 0, rows = 0, cols = 0;
+  bool alive = false;
+};
+
+// Scratch of one batch in flight.
+struct Scratch {
+  DevBuf recs, tile_count, tile_off, tile_cursor, bins, geom_status,
+      scan_status, counters, draws, jobs, outputs;
+  HostBuf h_draws, h_jobs, h_counters;
+};
+
+// Jobs and draws of a batch, as the kernels see them.
+struct BatchPlan {
+  std::vector<JobDesc> jobs;
+  std::vector<DrawDesc> draws;
+  int64_t total_tris = 0;
+  int64_t ntiles = 0;
+};
+
+}  // namespace mr
+
+struct mr_ctx {
+  int32_t device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0;
+  uint64_t resident_bytes = 0;
+  uint64_t scratch_bytes = 0;
+  std::vector<mr::MeshRec> meshes;
+  std::vector<mr::TexRec> textures;
+  mr::Scratch sc;
+  int32_t tile_w = 16, tile_h = 16;
+  float bin_factor = 3.0f;
+  bool profiling = false;
+  mr_stats stats{};
+  hipEvent_t ev[6] = {};
+  int32_t num_cus = 256;
+  std::recursive_mutex mu;
+};
+
+namespace mr {
+int dev_alloc(mr_ctx* c, DevBuf& b, size_t bytes, bool scratch);
+void dev_free(mr_ctx* c, DevBuf& b, bool scratch);
+int host_alloc(HostBuf& b, size_t bytes);
+void host_free(HostBuf& b);
+int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan);
+BatchArgs make_args(mr_ctx* c, const BatchPlan& p);
+int launch_batch(mr_ctx* c, const BatchArgs& a, bool profile);
+int run_batch(mr_ctx* c, BatchPlan& p);
+int render_frames(mr_ctx* c, const mr_frame* frames, int n, const mr_outputs* outs);
+}  // namespace mr

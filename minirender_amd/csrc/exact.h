@@ -1,0 +1,307 @@
+// exact.h -- fp32 arithmetic of the rasterizer, shared by every kernel.
+//
+// Each function states the reference lines (aslze/minirender
+// src/Renderer.cpp) whose operation order it keeps.  Device code is built
+// with -ffp-contract=off: every a*b+c below is two rounded operations, as on
+// the reference's x86-64 SSE build.  Division and sqrt are IEEE correctly
+// rounded on gfx950 (hipcc default), fp32 denormals are preserved.
+//
+// The functions are plain inline code so a CPU build (g++) of this header can
+// check them against straightforward serial loops.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "common.h"
+
+namespace mr {
+
+struct f2 { float x, y; };
+struct f3 { float x, y, z; };
+struct Vtx { f3 p, n; f2 uv; };  // Vertex, Scene.h:14-24
+
+MR_HD inline uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
+MR_HD inline float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+MR_HD inline double pow2d(int e) {  // exact 2^e for -1022 <= e <= 1023
+  return __builtin_bit_cast(double, (uint64_t)(1023 + e) << 52);
+}
+
+// ASL min/max/clamp: a<b?a:b, a>b?a:b, min(max(x,a),b).
+MR_HD inline float amin(float a, float b) { return a < b ? a : b; }
+MR_HD inline float amax(float a, float b) { return a > b ? a : b; }
+MR_HD inline float aclamp(float x, float a, float b) { return amin(amax(x, a), b); }
+
+MR_HD inline f3 add3(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+MR_HD inline f3 sub3(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+MR_HD inline f3 scale3(float k, f3 a) { return f3{k * a.x, k * a.y, k * a.z}; }
+MR_HD inline float dot3(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+MR_HD inline float len3(f3 a) { return sqrtf(dot3(a, a)); }
+MR_HD inline f3 norm3(f3 a) {
+  float l = len3(a);
+  return f3{a.x / l, a.y / l, a.z / l};
+}
+
+// ASL Matrix4 * Vec3, affine, rows 0..2 of M (Renderer.cpp:345,348).
+MR_HD inline f3 xform(const float* M, f3 p) {
+  return f3{((M[0] * p.x + M[1] * p.y) + M[2] * p.z) + M[3],
+            ((M[4] * p.x + M[5] * p.y) + M[6] * p.z) + M[7],
+            ((M[8] * p.x + M[9] * p.y) + M[10] * p.z) + M[11]};
+}
+
+// htransform, Renderer.cpp:13-20.
+MR_HD inline f3 htransform(const float* M, f3 p) {
+  float iw = 1.0f / (((M[12] * p.x + M[13] * p.y) + M[14] * p.z) + M[15]);
+  return f3{(((M[0] * p.x + M[1] * p.y) + M[2] * p.z) + M[3]) * iw,
+            (((M[4] * p.x + M[5] * p.y) + M[6] * p.z) + M[7]) * iw,
+            (((M[8] * p.x + M[9] * p.y) + M[10] * p.z) + M[11]) * iw};
+}
+
+// Renderer.cpp:324-326.
+MR_HD inline float znear_of(const float* P) {
+  bool persp = P[15] == 0.0f;
+  float zn = persp ? P[11] / (P[10] - 1.0f) : (P[11] + 1.0f) / P[10];
+  return -zn;
+}
+
+// ---------------------------------------------------------------------------
+// Exact repetition of the edge-walk recurrence e += n.x (Renderer.cpp:243).
+//
+// ff_add(s, d, n) returns the value of `for (i < n) s = s + d;` in IEEE fp32
+// round-to-nearest-even, without doing n dependent adds.  While s and the
+// exact sum s + d stay inside one binade [2^E, 2^(E+1)) (scaled to units of
+// u = 2^(E-23), s is an integer S in [2^23, 2^24)), every step adds the same
+// integer D = round(d/u) -- unless d/u is exactly half way between integers
+// (then the rounding depends on S's parity).  So the loop jumps over every
+// run of such steps in one multiply, and takes single IEEE steps where the
+// premise fails: zero/subnormal/inf/NaN operands, |d| >= 2^E, ties.  All
+// double arithmetic below is exact (operands need < 53 bits) except the one
+// division, whose result is corrected by exact comparisons.
+// ---------------------------------------------------------------------------
+MR_HD inline float ff_add(float s, float d, int n) {
+  while (n > 0) {
+    uint32_t sb = f2u(s);
+    uint32_t se = (sb >> 23) & 0xFFu;
+    uint32_t db = f2u(d);
+    if (se == 0u || se == 0xFFu || ((db >> 23) & 0xFFu) == 0xFFu) {
+      s = s + d;  // zero / subnormal / inf / NaN: one IEEE step
+      --n;
+      continue;
+    }
+    if ((db & 0x7FFFFFFFu) == 0u) return s;  // s + (+-0) == s for s != 0
+    const int E = (int)se - 127;
+    const int64_t S = (int64_t)((sb & 0x7FFFFFu) | 0x800000u);
+    double q = (double)d * pow2d(23 - E);  // d in units of u (exact)
+    if (sb >> 31) q = -q;                   // mirror: work on |s|
+    if (fabs(q) >= 8388608.0) {             // |d| >= 2^E: may leave the binade
+      s = s + d;
+      --n;
+      continue;
+    }
+    const double fq = floor(q);
+    if (q - fq == 0.5) {  // tie: rounding depends on the parity of S
+      s = s + d;
+      --n;
+      continue;
+    }
+    const int64_t D = (int64_t)rint(q);
+    if (D == 0) {
+      // |d| < u/2: s is a fixed point of s = s + d unless s + d drops below
+      // 2^E, where the spacing halves (s a power of two stepping down).
+      if ((double)S + q >= 8388608.0) return s;
+      s = s + d;
+      --n;
+      continue;
+    }
+    int64_t m;
+    if (D > 0) {  // step k valid while S + k*D + q < 2^24
+      const double R = (16777216.0 - (double)S) - q;
+      if (R <= 0.0) {
+        m = 0;
+      } else {
+        m = (int64_t)ceil(R / (double)D);
+        while (m > 0 && !((double)(S + (m - 1) * D) + q < 16777216.0)) --m;
+        while ((double)(S + m * D) + q < 16777216.0) ++m;
+      }
+    } else {  // step k valid while S + k*D + q >= 2^23
+      const double R = ((double)S + q) - 8388608.0;
+      if (R < 0.0) {
+        m = 0;
+      } else {
+        m = (int64_t)floor(R / (double)(-D)) + 1;
+        while (m > 0 && !((double)(S + (m - 1) * D) + q >= 8388608.0)) --m;
+        while ((double)(S + m * D) + q >= 8388608.0) ++m;
+      }
+    }
+    if (m == 0) {
+      s = s + d;
+      --n;
+      continue;
+    }
+    if (m > n) m = n;
+    const float r = (float)((double)(S + m * D) * pow2d(E - 23));
+    s = (sb >> 31) ? -r : r;
+    n -= (int)m;
+  }
+  return s;
+}
+
+// s after n steps of s = s + d: serial for short runs, ff_add otherwise.
+MR_HD inline float advance(float s, float d, int n) {
+  if (n <= 24) {
+    for (int i = 0; i < n; i++) s = s + d;
+    return s;
+  }
+  return ff_add(s, d, n);
+}
+
+// Row start of the edge walk at the bbox's left column (Renderer.cpp:240-242).
+// x0f = floor(pmin.x) + 0.5f, y = row + 0.5f.
+MR_HD inline void edge_start(float n1x, float n1y, float n2x, float n2y,
+                             float p0x, float p0y, float p2x, float p2y,
+                             float x0f, float y, float& e1, float& e2) {
+  e1 = n1x * (x0f - p2x) + n1y * (y - p2y);
+  e2 = n2x * (x0f - p0x) + n2y * (y - p0y);
+}
+
+// Coverage (Renderer.cpp:245): inclusive edges, no tie rule.
+MR_HD inline bool covered(float e1, float e2) {
+  return !(e1 < 0.0f || e2 < 0.0f || (1.0f - e1) - e2 < 0.0f);
+}
+
+// Depth and perspective-corrected weights (Renderer.cpp:247-261).
+MR_HD inline float depth_of(float e1, float e2, float z0, float z1, float z2,
+                            bool persp, float k[3]) {
+  k[0] = (1.0f - e1) - e2;
+  k[1] = e1;
+  k[2] = e2;
+  if (persp) {
+    float z = 1.0f / ((k[0] * z0 + k[1] * z1) + k[2] * z2);
+    k[0] = k[0] * (z0 * z);
+    k[1] = k[1] * (z1 * z);
+    k[2] = k[2] * (z2 * z);
+    return z;
+  }
+  return ((k[0] * z0 + k[1] * z1) + k[2] * z2) + 0.0f * 1.0f;  // + k[3]*zz[3]
+}
+
+// Order-preserving 32-bit key of a depth (NaN excluded by the caller):
+// -0 and +0 share a key so that equal depths tie-break on draw order, as the
+// reference's strict `z < depth` does (Renderer.cpp:267).
+MR_HD inline uint32_t depth_key(float z) {
+  uint32_t b = f2u(z);
+  if (b == 0x80000000u) b = 0u;
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// clip, Renderer.cpp:121-129.
+MR_HD inline Vtx clip_edge(float z, const Vtx& a, const Vtx& b) {
+  float d = b.p.z - a.p.z;
+  float k = (fabsf(d) < 1e-6f) ? 0.5f : (z - a.p.z) / d;
+  float k1 = 1.0f - k;
+  Vtx v;
+  v.p = add3(scale3(k, b.p), scale3(k1, a.p));
+  v.n = add3(scale3(k, b.n), scale3(k1, a.n));
+  v.uv.x = k * b.uv.x + k1 * a.uv.x;
+  v.uv.y = k * b.uv.y + k1 * a.uv.y;
+  return v;
+}
+
+// clipTriangle, Renderer.cpp:131-161, for a triangle with at least one vertex
+// beyond the near plane and not all three.  Writes the sub-triangles in draw
+// order into out[0..2] and out[3..5]; returns their number (1 or 2).
+MR_HD inline int clip_triangle(float z, Vtx v0, Vtx v1, Vtx v2, Vtx out[6]) {
+  while (v0.p.z < v1.p.z || v0.p.z < v2.p.z) {  // rotate (a,b,c) -> (c,a,b)
+    Vtx t = v0; v0 = v2; v2 = v1; v1 = t;
+  }
+  if (v1.p.z > z) {
+    out[0] = clip_edge(z, v0, v2);
+    out[1] = clip_edge(z, v1, v2);
+    out[2] = v2;
+    return 1;
+  } else if (v2.p.z > z) {
+    out[0] = clip_edge(z, v0, v1);
+    out[1] = v1;
+    out[2] = clip_edge(z, v1, v2);
+    return 1;
+  }
+  Vtx v01 = clip_edge(z, v0, v1);
+  Vtx v02 = clip_edge(z, v0, v2);
+  out[0] = v01; out[1] = v1; out[2] = v2;
+  out[3] = v01; out[4] = v2; out[5] = v02;
+  return 2;
+}
+
+// Triangle setup of paintTriangle (Renderer.cpp:179-236) for eye-space
+// positions.  Returns false when the triangle is rejected off-screen or
+// back-face culled.  Fills the raster record fields.
+MR_HD inline bool setup_triangle(const float* P, bool persp, float w, float h,
+                                 f3 a, f3 b, f3 c, TriRec& r) {
+  f3 ndc0 = htransform(P, a), ndc1 = htransform(P, b), ndc2 = htransform(P, c);
+  const float hw = w / 2.0f, hh = h / 2.0f;
+  float p0x = (1.0f + ndc0.x) * hw, p0y = (1.0f - ndc0.y) * hh;
+  float p1x = (1.0f + ndc1.x) * hw, p1y = (1.0f - ndc1.y) * hh;
+  float p2x = (1.0f + ndc2.x) * hw, p2y = (1.0f - ndc2.y) * hh;
+  float minx = amin(amin(amin(1e30f, p0x), p1x), p2x);
+  float miny = amin(amin(amin(1e30f, p0y), p1y), p2y);
+  float maxx = amax(amax(amax(-1e30f, p0x), p1x), p2x);
+  float maxy = amax(amax(amax(-1e30f, p0y), p1y), p2y);
+  if (maxx < 0 || maxy < 0 || minx > w || miny > h) return false;
+  float ar = (p0x - p1x) * (p2y - p1y) - (p0y - p1y) * (p2x - p1x);
+  if (!(ar > 0.0f)) {
+    // a <= 0 is culled (:207); a NaN passes the cull in the reference but
+    // every edge value is then NaN and so is z: nothing is written.
+    return false;
+  }
+  float i2a = -1.0f / ar;
+  r.n1x = -(p0y - p2y) * i2a;
+  r.n1y = (p0x - p2x) * i2a;
+  r.n2x = -(p1y - p0y) * i2a;
+  r.n2y = (p1x - p0x) * i2a;
+  r.p0x = p0x; r.p0y = p0y; r.p2x = p2x; r.p2y = p2y;
+  minx = aclamp(minx, 0.0f, w - 1);
+  maxx = aclamp(maxx, 0.0f, w - 1);
+  miny = aclamp(miny, 0.0f, h - 1);
+  maxy = aclamp(maxy, 0.0f, h - 1);
+  r.bx0 = (uint16_t)(int)floorf(minx);
+  r.bx1 = (uint16_t)(int)floorf(maxx);
+  r.by0 = (uint16_t)(int)floorf(miny);
+  r.by1 = (uint16_t)(int)floorf(maxy);
+  if (persp) {
+    r.z0 = -1.0f / a.z;
+    r.z1 = -1.0f / b.z;
+    r.z2 = -1.0f / c.z;
+  } else {
+    r.z0 = ndc0.z;
+    r.z1 = ndc1.z;
+    r.z2 = ndc2.z;
+  }
+  return true;
+}
+
+// fract + nearest texel index of Renderer.cpp:274, clamped to n-1 where the
+// reference would read one past the end (fract rounding to 1.0f).
+MR_HD inline int tex_index(float u, int n) {
+  float fr = u - floorf(u);
+  float v = fr * (float)n;
+  int i = (v >= 0.0f && v < 2147483648.0f) ? (int)v : 0;
+  return i > n - 1 ? n - 1 : i;
+}
+
+MR_HD inline f3 texel(const void* tex, int fmt, int cols, int i, int j) {
+  size_t o = ((size_t)i * (size_t)cols + (size_t)j) * 3;
+  if (fmt == 0) {  // RGB8: byte / 255 (io.cpp:405-411)
+    const uint8_t* p = (const uint8_t*)tex + o;
+    return f3{(float)p[0] / 255.0f, (float)p[1] / 255.0f, (float)p[2] / 255.0f};
+  }
+  const float* p = (const float*)tex + o;
+  return f3{p[0], p[1], p[2]};
+}
+
+// savePPM quantization, io.cpp:356-361.
+MR_HD inline uint8_t quantize(float v) {
+  return (uint8_t)aclamp(v * 255.0f, 0.0f, 255.0f);
+}
+
+}  // namespace mr

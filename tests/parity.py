@@ -1,0 +1,87 @@
+"""Parity helpers shared by the GPU tests: the HIP path against the oracle.
+
+Bar: winner (draw-order index of the visible triangle) and depth bits equal,
+RGB8 colour within 1 LSB per channel, unquantized colour within 1e-5 relative
+(powf of libm vs the device library), normals bit-equal.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from minirender_amd import scenes
+from minirender_amd.scenes import Draw, FrameSpec, Material
+
+RGB8_TOL = 1
+
+
+def assert_parity(gpu: dict, ora: dict, what: str = "", normals: bool = True) -> None:
+    """normals: compare the normals image (defined only with save_normals)."""
+    if "winner" in gpu:
+        bad = np.argwhere(gpu["winner"] != ora["winner"])
+        assert bad.size == 0, (
+            f"{what}: {len(bad)} winner mismatches, first at {tuple(bad[0])}: "
+            f"gpu {gpu['winner'][tuple(bad[0])]:#x} oracle {ora['winner'][tuple(bad[0])]:#x}")
+    if "depth" in gpu:
+        gb, ob = gpu["depth"].view(np.uint32), ora["depth"].view(np.uint32)
+        bad = np.argwhere(gb != ob)
+        assert bad.size == 0, (
+            f"{what}: {len(bad)} depth mismatches, first at {tuple(bad[0])}: "
+            f"gpu {gpu['depth'][tuple(bad[0])]!r} oracle {ora['depth'][tuple(bad[0])]!r}")
+    if "rgb8" in gpu:
+        d = np.abs(gpu["rgb8"].astype(np.int32) - ora["rgb8"].astype(np.int32))
+        assert d.max(initial=0) <= RGB8_TOL, f"{what}: rgb8 max diff {d.max()}"
+    if "rgb" in gpu:
+        g, o = gpu["rgb"], ora["rgb"]
+        both_nan = np.isnan(g) & np.isnan(o)
+        close = np.isclose(g, o, rtol=1e-5, atol=1e-6) | both_nan
+        assert close.all(), f"{what}: float colour differs at {np.argwhere(~close)[0]}"
+    if "normals" in gpu and normals:
+        assert np.array_equal(gpu["normals"].view(np.uint32), ora["normals"].view(np.uint32)), \
+            f"{what}: normals differ"
+
+
+def fuzz_frame(seed: int, width: int = 97, height: int = 61) -> FrameSpec:
+    """A seeded small scene: spheres / cubes / cylinders / random triangles
+    / a textured height-field, random camera that may sit inside the scene
+    (near clipping), perspective or orthographic, point or directional
+    light, lighting and texturing flags."""
+    rng = np.random.default_rng(seed)
+    draws = []
+    prims = [scenes.create_sphere(1.0, int(rng.integers(3, 12)), int(rng.integers(3, 20))),
+             scenes.create_cube(1.0), scenes.create_cylinder(0.5, 1.0, int(rng.integers(3, 16)), 2)]
+    tex = scenes.checker_texture(64, seed, 8)
+    for _ in range(int(rng.integers(1, 6))):
+        kind = int(rng.integers(0, 5))
+        if kind < 3:
+            mesh = prims[kind]
+        elif kind == 3:
+            mesh = scenes.random_triangles(int(rng.integers(5, 60)), int(rng.integers(1 << 30)),
+                                           lo=0.05, hi=1.5, center_lo=(-1, -1, -1), center_hi=(1, 1, 1))
+        else:
+            mesh = scenes.height_field(int(rng.integers(3, 24)), int(rng.integers(1 << 30)), 0.5, uv=True)
+        xf = (scenes.translate(*rng.uniform(-1, 1, 3)) @
+              scenes.rotate(rng.normal(size=3), float(rng.uniform(0, 6.3))) @
+              scenes.scale(float(rng.uniform(0.3, 1.5))))
+        mat = Material(diffuse=tuple(rng.uniform(0, 1, 3)), specular=tuple(rng.uniform(0, 1, 3)),
+                       emissive=tuple(rng.uniform(0, 0.2, 3)),
+                       shininess=float(rng.choice([0.0, 1.0, 12.0, 40.0])),
+                       texture=tex if rng.random() < 0.4 else None)
+        draws.append(Draw(mesh, mat, xf.astype(np.float32)))
+    aspect = width / height
+    if rng.random() < 0.75:
+        proj = scenes.projection_frustum(math.radians(float(rng.uniform(30, 90))), aspect,
+                                         float(rng.uniform(0.05, 0.5)), 50.0)
+    else:
+        s = float(rng.uniform(1.5, 4))
+        proj = scenes.projection_ortho(-s * aspect, s * aspect, -s, s, 0.1, 20.0)
+    r = float(rng.uniform(0.3, 4.0))  # small r: camera inside the scene
+    th, ph = float(rng.uniform(0, 6.3)), float(rng.uniform(-1.2, 1.2))
+    eye = (r * math.cos(ph) * math.cos(th), r * math.sin(ph), r * math.cos(ph) * math.sin(th))
+    view = scenes.look_at(eye, tuple(rng.uniform(-0.3, 0.3, 3)))
+    return FrameSpec(width, height, proj, view=view, draws=draws,
+                     light=tuple(rng.uniform(-3, 3, 3)), light_is_point=bool(rng.random() < 0.5),
+                     background=tuple(rng.uniform(0, 1, 3)), ambient=float(rng.uniform(0, 0.3)),
+                     lighting=bool(rng.random() < 0.85), texturing=bool(rng.random() < 0.8),
+                     save_normals=bool(rng.random() < 0.5))
