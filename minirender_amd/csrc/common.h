@@ -56,48 +56,50 @@ struct JobDesc {
   float ambient;
   int32_t lighting;
   int32_t save_normals;
-  int32_t pad0;
-  uint8_t* rgb8;     // device outputs (any may be null)
+  int32_t shade_base;  // first shading block of this job
+  uint32_t* win;       // device scratch: draw-order id of each pixel's winner
+  uint8_t* rgb8;       // device outputs (any may be null)
   float* depth;
   uint32_t* winner;
   float* rgb;
   float* normals;
 };
 
-// Raster record of one surviving (sub-)triangle: 64 bytes, written by the
-// geometry kernel in draw order (order-preserving compaction), so its index
-// is a draw-order key for the depth tie-break.
-struct TriRec {
-  float n1x, n1y, n2x, n2y;  // edge normals, Renderer.cpp:215-216
-  float p0x, p0y, p2x, p2y;  // pixel positions of vertices 0 and 2
-  float z0, z1, z2;          // -1/z_eye (perspective) or NDC z (ortho)
-  uint32_t tri_sub;          // tri << 2 | clipped << 1 | sub
-  uint16_t bx0, bx1, by0, by1;  // clamped pixel bbox (inclusive), bytes 48..55
-  uint32_t draw;             // index into the batch's draw table
-  uint32_t job;              // bytes 60..63: the binner reads 48..63 as one uint4
+// A surviving (sub-)triangle as the binner sees it: its draw-order id
+// (2 * flat triangle index + clip sub-index) and clamped pixel bbox.
+struct BinItem {
+  uint32_t id;
+  uint32_t job;
+  uint16_t bx0, bx1, by0, by1;
 };
-static_assert(sizeof(TriRec) == 64, "TriRec must be 64 bytes");
+static_assert(sizeof(BinItem) == 16, "BinItem must be 16 bytes");
 
 // Device counters of a batch (64-bit, cleared by the clear kernel).
 enum Counter : int {
-  kGeomTicket = 0,
+  kItems = 0,        // surviving (sub-)triangles (statistics)
   kScanTicket = 1,
-  kRecords = 2,      // total records (written by the last geometry chunk)
-  kBinOverflow = 3,  // set when the bin list was too small
-  kError = 4,        // set when a bounded wait hit its limit
-  kClipped = 5,      // input triangles that crossed the near plane
-  kTileTicket = 6,
-  kBinEntries = 7,   // total (tile, record) pairs (written by the scan)
+  kBinOverflow = 2,  // set when the bin list was too small
+  kError = 3,        // set when a bounded wait hit its limit
+  kClipped = 4,      // input triangles that crossed the near plane
+  kTileTicket = 5,
+  kBinEntries = 6,   // total (tile, triangle) pairs (written by the scan)
+  kParts = 7,        // work items of split tiles (written by the scan)
+  kHeavy = 8,        // tiles split over several work items
+  kHeavyOverflow = 9,
   kNumCounters = 16
 };
+
+constexpr int kRegion = 128;       // item slots per geometry wavefront (64 x 2)
+constexpr int kShadeBlock = 256;   // pixels per deferred-shading workgroup
+constexpr int kMaxLdsDraws = 256;  // draw-table entries cached in LDS
 
 // Bounded waits (decoupled look-back): polls before giving up with kError.
 constexpr uint32_t kSpinLimit = 1u << 22;
 
-constexpr int kGeomChunk = 256;  // triangles per geometry chunk (1 per thread)
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;    // elements per thread in the tile-count scan
 constexpr uint32_t kNoWinner = 0xFFFFFFFFu;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 constexpr float kClearDepth = 1e11f;  // Renderer.cpp:116
 constexpr int kMaxDim = 16384;        // max width / height
 

@@ -95,8 +95,12 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes,
                 BatchPlan& plan) {
   plan.jobs.assign(nframes, JobDesc{});
   plan.draws.clear();
+  plan.draw_begin.clear();
+  plan.job_bases.assign(2 * (size_t)nframes, 0);
   plan.total_tris = 0;
   plan.ntiles = 0;
+  plan.npixels = 0;
+  plan.shade_blocks = 0;
   const int tw = c->tile_w, th = c->tile_h;
   for (int j = 0; j < nframes; j++) {
     const mr_frame* f = &frames[j];
@@ -115,6 +119,13 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes,
     if ((int64_t)plan.ntiles + (int64_t)J.tiles_x * J.tiles_y > 0x7FFFFFFF)
       return MR_ERR_ARG;
     plan.ntiles += J.tiles_x * J.tiles_y;
+    plan.job_bases[j] = J.tile_base;
+    const int64_t npix = (int64_t)f->width * f->height;
+    J.shade_base = (int32_t)plan.shade_blocks;
+    plan.job_bases[nframes + j] = J.shade_base;
+    plan.shade_blocks += (npix + kShadeBlock - 1) / kShadeBlock;
+    plan.npixels += npix;
+    if (plan.shade_blocks > 0x7FFFFFFF) return MR_ERR_ARG;
     J.draw_begin = (int32_t)plan.draws.size();
     for (int i = 0; i < 3; i++) {
       J.bg[i] = f->background[i];
@@ -167,6 +178,7 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes,
       order += 2ull * (uint64_t)m.ntri;
       plan.total_tris += m.ntri;
       plan.draws.push_back(D);
+      plan.draw_begin.push_back(D.tri_begin);
     }
     if (order >= 0xFFFFFFFFull) return MR_ERR_ARG;
     J.draw_end = (int32_t)plan.draws.size();
@@ -175,23 +187,38 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes,
   return MR_OK;
 }
 
-static int ensure_scratch(mr_ctx* c, const BatchPlan& p, uint64_t bins_cap) {
+static int ensure_scratch(mr_ctx* c, const BatchPlan& p, uint64_t bins_cap,
+                          uint32_t heavy_slots) {
   Scratch& s = c->sc;
   const int64_t T = p.total_tris;
-  const int gch = geom_chunks_for(T), scb = scan_blocks_for(p.ntiles);
+  const int scb = scan_blocks_for((int32_t)p.ntiles);
+  const size_t nt = (size_t)p.ntiles * 4;
+  const size_t nd = std::max<size_t>(p.draws.size(), 1);
+  const int64_t nregions = (T + 63) / 64;
+  const size_t npt = (size_t)c->tile_w * c->tile_h * 8;
+  const size_t parts = (size_t)(bins_cap / c->frag_size) + heavy_slots + 16;
   int rc;
-  if ((rc = dev_alloc(c, s.recs, (size_t)(2 * T) * sizeof(TriRec), true))) return rc;
-  if ((rc = dev_alloc(c, s.tile_count, (size_t)p.ntiles * 4, true))) return rc;
-  if ((rc = dev_alloc(c, s.tile_off, (size_t)p.ntiles * 4, true))) return rc;
-  if ((rc = dev_alloc(c, s.tile_cursor, (size_t)p.ntiles * 4, true))) return rc;
+  if ((rc = dev_alloc(c, s.items, (size_t)std::max<int64_t>(nregions, 1) * kRegion * sizeof(BinItem), true))) return rc;
+  if ((rc = dev_alloc(c, s.region_count, (size_t)std::max<int64_t>(nregions, 1) * 4, true))) return rc;
+  if ((rc = dev_alloc(c, s.tile_count, nt, true))) return rc;
+  if ((rc = dev_alloc(c, s.tile_off, nt, true))) return rc;
+  if ((rc = dev_alloc(c, s.tile_cursor, nt, true))) return rc;
+  if ((rc = dev_alloc(c, s.tile_heavy, nt, true))) return rc;
+  if ((rc = dev_alloc(c, s.tile_done, nt, true))) return rc;
+  if ((rc = dev_alloc(c, s.parts, parts * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.bins, (size_t)bins_cap * 4, true))) return rc;
-  if ((rc = dev_alloc(c, s.geom_status, (size_t)std::max(gch, 1) * 8, true))) return rc;
+  if ((rc = dev_alloc(c, s.heavy_keys, (size_t)heavy_slots * npt, true))) return rc;
   if ((rc = dev_alloc(c, s.scan_status, (size_t)std::max(scb, 1) * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.counters, kNumCounters * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.draws, std::max<size_t>(p.draws.size(), 1) * sizeof(DrawDesc), true))) return rc;
+  if ((rc = dev_alloc(c, s.draws, nd * sizeof(DrawDesc), true))) return rc;
+  if ((rc = dev_alloc(c, s.draw_begin, nd * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.jobs, p.jobs.size() * sizeof(JobDesc), true))) return rc;
-  if ((rc = host_alloc(s.h_draws, std::max<size_t>(p.draws.size(), 1) * sizeof(DrawDesc)))) return rc;
+  if ((rc = dev_alloc(c, s.job_bases, p.job_bases.size() * 4, true))) return rc;
+  if ((rc = dev_alloc(c, s.win, (size_t)p.npixels * 4, true))) return rc;
+  if ((rc = host_alloc(s.h_draws, nd * sizeof(DrawDesc)))) return rc;
+  if ((rc = host_alloc(s.h_draw_begin, nd * 8))) return rc;
   if ((rc = host_alloc(s.h_jobs, p.jobs.size() * sizeof(JobDesc)))) return rc;
+  if ((rc = host_alloc(s.h_job_bases, p.job_bases.size() * 4))) return rc;
   if ((rc = host_alloc(s.h_counters, kNumCounters * 8))) return rc;
   return MR_OK;
 }
@@ -200,26 +227,37 @@ BatchArgs make_args(mr_ctx* c, const BatchPlan& p) {
   Scratch& s = c->sc;
   BatchArgs a;
   memset(&a, 0, sizeof(a));
-  a.recs = (TriRec*)s.recs.p;
+  a.items = (BinItem*)s.items.p;
+  a.region_count = (uint32_t*)s.region_count.p;
+  a.nregions = (p.total_tris + 63) / 64;
   a.tile_count = (uint32_t*)s.tile_count.p;
   a.tile_off = (uint32_t*)s.tile_off.p;
   a.tile_cursor = (uint32_t*)s.tile_cursor.p;
+  a.tile_heavy = (uint32_t*)s.tile_heavy.p;
+  a.tile_done = (uint32_t*)s.tile_done.p;
+  a.parts = (uint2*)s.parts.p;
+  a.parts_cap = (uint32_t)std::min<size_t>(s.parts.bytes / 8, 0xFFFFFFF0u);
   a.bins = (uint32_t*)s.bins.p;
   a.bins_cap = s.bins.bytes / 4;
   if (a.bins_cap > 0xFFFFFFF0ull) a.bins_cap = 0xFFFFFFF0ull;
-  a.geom_status = (unsigned long long*)s.geom_status.p;
+  a.heavy_keys = (unsigned long long*)s.heavy_keys.p;
+  a.heavy_cap = (uint32_t)(s.heavy_keys.bytes / ((size_t)c->tile_w * c->tile_h * 8));
+  a.frag_size = c->frag_size;
   a.scan_status = (unsigned long long*)s.scan_status.p;
   a.counters = (unsigned long long*)s.counters.p;
   a.draws = (const DrawDesc*)s.draws.p;
+  a.draw_begin = (const int64_t*)s.draw_begin.p;
   a.ndraws = (int32_t)p.draws.size();
   a.jobs = (const JobDesc*)s.jobs.p;
+  a.job_tile_base = (const int32_t*)s.job_bases.p;
+  a.job_shade_base = (const int32_t*)s.job_bases.p + p.jobs.size();
   a.njobs = (int32_t)p.jobs.size();
   a.total_tris = p.total_tris;
   a.ntiles = (int32_t)p.ntiles;
   a.tile_w = c->tile_w;
   a.tile_h = c->tile_h;
-  a.geom_chunks = geom_chunks_for(p.total_tris);
   a.scan_blocks = scan_blocks_for((int32_t)p.ntiles);
+  a.shade_blocks = (int32_t)p.shade_blocks;
   a.num_cus = c->num_cus;
   return a;
 }
@@ -238,6 +276,8 @@ int launch_batch(mr_ctx* c, const BatchArgs& a, bool profile) {
   if (profile) HIP_TRY(hipEventRecord(c->ev[4], st));
   HIP_TRY(launch_tile(a, st));
   if (profile) HIP_TRY(hipEventRecord(c->ev[5], st));
+  HIP_TRY(launch_shade(a, st));
+  if (profile) HIP_TRY(hipEventRecord(c->ev[6], st));
   return MR_OK;
 }
 
@@ -247,16 +287,30 @@ int run_batch(mr_ctx* c, BatchPlan& p) {
   uint64_t want_bins = std::max<uint64_t>(
       (uint64_t)((double)p.total_tris * c->bin_factor) + (uint64_t)p.ntiles, 1u << 16);
   if (c->sc.bins.bytes / 4 > want_bins) want_bins = c->sc.bins.bytes / 4;
-  int rc = ensure_scratch(c, p, want_bins);
+  const uint32_t want_heavy = std::max<uint32_t>(
+      c->heavy_slots, (uint32_t)std::min<uint64_t>(want_bins / c->frag_size + 16, 1u << 20));
+  int rc = ensure_scratch(c, p, want_bins, want_heavy);
   if (rc) return rc;
   Scratch& s = c->sc;
+  uint32_t* win = (uint32_t*)s.win.p;
+  for (auto& J : p.jobs) {
+    J.win = win;
+    win += (size_t)J.W * J.H;
+  }
   memcpy(s.h_jobs.p, p.jobs.data(), p.jobs.size() * sizeof(JobDesc));
-  if (!p.draws.empty()) memcpy(s.h_draws.p, p.draws.data(), p.draws.size() * sizeof(DrawDesc));
   HIP_TRY(hipMemcpyAsync(s.jobs.p, s.h_jobs.p, p.jobs.size() * sizeof(JobDesc),
                          hipMemcpyHostToDevice, c->stream));
-  if (!p.draws.empty())
+  memcpy(s.h_job_bases.p, p.job_bases.data(), p.job_bases.size() * 4);
+  HIP_TRY(hipMemcpyAsync(s.job_bases.p, s.h_job_bases.p, p.job_bases.size() * 4,
+                         hipMemcpyHostToDevice, c->stream));
+  if (!p.draws.empty()) {
+    memcpy(s.h_draws.p, p.draws.data(), p.draws.size() * sizeof(DrawDesc));
+    memcpy(s.h_draw_begin.p, p.draw_begin.data(), p.draws.size() * 8);
     HIP_TRY(hipMemcpyAsync(s.draws.p, s.h_draws.p, p.draws.size() * sizeof(DrawDesc),
                            hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(s.draw_begin.p, s.h_draw_begin.p, p.draws.size() * 8,
+                           hipMemcpyHostToDevice, c->stream));
+  }
   unsigned long long* hc = (unsigned long long*)s.h_counters.p;
   c->stats.retries = 0;
   for (int attempt = 0;; attempt++) {
@@ -265,24 +319,33 @@ int run_batch(mr_ctx* c, BatchPlan& p) {
     HIP_TRY(hipMemcpyAsync(hc, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (hc[kError]) return MR_ERR_DEVICE;
-    if (!hc[kBinOverflow]) break;
+    if (!hc[kBinOverflow] && !hc[kHeavyOverflow]) break;
     if (attempt >= 3 || hc[kBinEntries] >= 0xFFFFFFF0ull) return MR_ERR_NOMEM;
     c->stats.retries++;
-    const uint64_t need = hc[kBinEntries] + hc[kBinEntries] / 4 + 1024;
-    if ((rc = dev_alloc(c, s.bins, (size_t)need * 4, true))) return rc;
+    if (hc[kBinOverflow]) {
+      const uint64_t need = hc[kBinEntries] + hc[kBinEntries] / 4 + 1024;
+      if ((rc = dev_alloc(c, s.bins, (size_t)need * 4, true))) return rc;
+    }
+    if (hc[kHeavyOverflow]) {
+      c->heavy_slots = (uint32_t)(hc[kHeavy] + hc[kHeavy] / 4 + 16);
+      if ((rc = dev_alloc(c, s.heavy_keys,
+                          (size_t)c->heavy_slots * c->tile_w * c->tile_h * 8, true)))
+        return rc;
+    }
   }
+  c->heavy_slots = std::max<uint32_t>(c->heavy_slots, (uint32_t)hc[kHeavy]);
   c->stats.triangles_in = p.total_tris;
   c->stats.triangles_clipped = (int64_t)hc[kClipped];
-  c->stats.records = (int64_t)hc[kRecords];
+  c->stats.records = (int64_t)hc[kItems];
   c->stats.bin_entries = (int64_t)hc[kBinEntries];
   c->stats.tiles = p.ntiles;
   if (c->profiling) {
     float ms = 0;
-    for (int i = 0; i < 5; i++) {
+    for (int i = 0; i < 6; i++) {
       hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]);
       c->stats.stage_ms[i] = ms;
     }
-    hipEventElapsedTime(&ms, c->ev[0], c->ev[5]);
+    hipEventElapsedTime(&ms, c->ev[0], c->ev[6]);
     c->stats.frame_ms = ms;
   }
   return MR_OK;
@@ -393,11 +456,15 @@ int mr_ctx_destroy(mr_ctx* c) {
   }
   for (auto& t : c->textures) dev_free(c, t.texels, false);
   Scratch& s = c->sc;
-  DevBuf* bufs[] = {&s.recs, &s.tile_count, &s.tile_off, &s.tile_cursor, &s.bins,
-                    &s.geom_status, &s.scan_status, &s.counters, &s.draws, &s.jobs, &s.outputs};
+  DevBuf* bufs[] = {&s.items, &s.region_count, &s.tile_count, &s.tile_off,
+                    &s.tile_cursor, &s.tile_heavy, &s.tile_done, &s.parts, &s.bins,
+                    &s.heavy_keys, &s.scan_status, &s.counters, &s.draws,
+                    &s.draw_begin, &s.jobs, &s.job_bases, &s.win, &s.outputs};
   for (DevBuf* b : bufs) dev_free(c, *b, true);
   host_free(s.h_draws);
+  host_free(s.h_draw_begin);
   host_free(s.h_jobs);
+  host_free(s.h_job_bases);
   host_free(s.h_counters);
   for (auto& e : c->ev) hipEventDestroy(e);
   hipStreamDestroy(c->own_stream);

@@ -40,17 +40,22 @@ struct TexRec {
 
 // Scratch of one batch in flight.
 struct Scratch {
-  DevBuf recs, tile_count, tile_off, tile_cursor, bins, geom_status,
-      scan_status, counters, draws, jobs, outputs;
-  HostBuf h_draws, h_jobs, h_counters;
+  DevBuf items, region_count, tile_count, tile_off, tile_cursor, tile_heavy,
+      tile_done, parts, bins, heavy_keys, scan_status, counters, draws,
+      draw_begin, jobs, job_bases, win, outputs;
+  HostBuf h_draws, h_draw_begin, h_jobs, h_job_bases, h_counters;
 };
 
 // Jobs and draws of a batch, as the kernels see them.
 struct BatchPlan {
   std::vector<JobDesc> jobs;
   std::vector<DrawDesc> draws;
+  std::vector<int64_t> draw_begin;
+  std::vector<int32_t> job_bases;  // tile bases of all jobs, then shade bases
   int64_t total_tris = 0;
   int64_t ntiles = 0;
+  int64_t npixels = 0;
+  int64_t shade_blocks = 0;
 };
 
 }  // namespace mr
@@ -67,9 +72,11 @@ struct mr_ctx {
   mr::Scratch sc;
   int32_t tile_w = 16, tile_h = 16;
   float bin_factor = 3.0f;
+  uint32_t frag_size = 1024;  // max (tile, triangle) pairs per raster item
+  uint32_t heavy_slots = 0;   // merge buffers of split tiles (grown on demand)
   bool profiling = false;
   mr_stats stats{};
-  hipEvent_t ev[6] = {};
+  hipEvent_t ev[7] = {};
   int32_t num_cus = 256;
   std::recursive_mutex mu;
 };

@@ -209,35 +209,88 @@ MR_HD inline Vtx clip_edge(float z, const Vtx& a, const Vtx& b) {
 }
 
 // clipTriangle, Renderer.cpp:131-161, for a triangle with at least one vertex
-// beyond the near plane and not all three.  Writes the sub-triangles in draw
-// order into out[0..2] and out[3..5]; returns their number (1 or 2).
-MR_HD inline int clip_triangle(float z, Vtx v0, Vtx v1, Vtx v2, Vtx out[6]) {
-  while (v0.p.z < v1.p.z || v0.p.z < v2.p.z) {  // rotate (a,b,c) -> (c,a,b)
-    Vtx t = v0; v0 = v2; v2 = v1; v1 = t;
-  }
-  if (v1.p.z > z) {
-    out[0] = clip_edge(z, v0, v2);
-    out[1] = clip_edge(z, v1, v2);
-    out[2] = v2;
-    return 1;
-  } else if (v2.p.z > z) {
-    out[0] = clip_edge(z, v0, v1);
-    out[1] = v1;
-    out[2] = clip_edge(z, v1, v2);
-    return 1;
-  }
-  Vtx v01 = clip_edge(z, v0, v1);
-  Vtx v02 = clip_edge(z, v0, v2);
-  out[0] = v01; out[1] = v1; out[2] = v2;
-  out[3] = v01; out[4] = v2; out[5] = v02;
+// beyond the near plane and not all three.  Returns the sub-triangles in draw
+// order, (a0,a1,a2) and (b0,b1,b2), and their number (1 or 2).
+// The reference rotates (a,b,c) -> (c,a,b) while v0 is not the farthest
+// (Renderer.cpp:136-140).  That stops after at most two rotations (also with
+// NaNs: some state always has a v0 that no other z exceeds), so the rotation
+// count is picked directly: 0 if a is farthest, else 1 if c is, else 2.
+MR_HD inline int rotations(float za, float zb, float zc) {
+  if (!(za < zb || za < zc)) return 0;
+  if (!(zc < za || zc < zb)) return 1;
   return 2;
 }
 
+MR_HD inline int clip_triangle(float z, Vtx v0, Vtx v1, Vtx v2, Vtx& a0, Vtx& a1,
+                               Vtx& a2, Vtx& b0, Vtx& b1, Vtx& b2) {
+  {
+    const int r = rotations(v0.p.z, v1.p.z, v2.p.z);
+    const Vtx A = v0, B = v1, C = v2;
+    v0 = r == 0 ? A : (r == 1 ? C : B);
+    v1 = r == 0 ? B : (r == 1 ? A : C);
+    v2 = r == 0 ? C : (r == 1 ? B : A);
+  }
+  // Every output is assigned on every path (keeps them in registers).
+  const bool beyond1 = v1.p.z > z, beyond2 = v2.p.z > z;
+  const Vtx p = clip_edge(z, v0, beyond1 ? v2 : v1);                 // v02 | v01
+  const Vtx q = clip_edge(z, beyond1 || beyond2 ? v1 : v0, v2);      // v12 | v02
+  if (beyond1) {         // v0, v1 beyond: (v02, v12, v2)
+    a0 = p; a1 = q; a2 = v2;
+  } else if (beyond2) {  // v0, v2 beyond: (v01, v1, v12)
+    a0 = p; a1 = v1; a2 = q;
+  } else {               // only v0 beyond: (v01, v1, v2), (v01, v2, v02)
+    a0 = p; a1 = v1; a2 = v2;
+  }
+  b0 = p; b1 = v2; b2 = q;
+  return (beyond1 || beyond2) ? 1 : 2;
+}
+
+// Position part of clip (Renderer.cpp:123,125): k depends on positions only,
+// so this equals clip_edge's position bit for bit.
+MR_HD inline f3 clip_pos(float z, f3 a, f3 b) {
+  float d = b.z - a.z;
+  float k = (fabsf(d) < 1e-6f) ? 0.5f : (z - a.z) / d;
+  return add3(scale3(k, b), scale3(1.0f - k, a));
+}
+
+// clipTriangle on positions only, for setup and binning.
+MR_HD inline int clip_positions(float z, f3 v0, f3 v1, f3 v2, f3& a0, f3& a1,
+                                f3& a2, f3& b0, f3& b1, f3& b2) {
+  {
+    const int r = rotations(v0.z, v1.z, v2.z);
+    const f3 A = v0, B = v1, C = v2;
+    v0 = r == 0 ? A : (r == 1 ? C : B);
+    v1 = r == 0 ? B : (r == 1 ? A : C);
+    v2 = r == 0 ? C : (r == 1 ? B : A);
+  }
+  // Every output is assigned on every path (keeps them in registers).
+  const bool beyond1 = v1.z > z, beyond2 = v2.z > z;
+  const f3 p = clip_pos(z, v0, beyond1 ? v2 : v1);    // v02 | v01
+  const f3 q = clip_pos(z, beyond1 || beyond2 ? v1 : v0, v2);  // v12 | v02
+  if (beyond1) {         // v0, v1 beyond: (v02, v12, v2)
+    a0 = p; a1 = q; a2 = v2;
+  } else if (beyond2) {  // v0, v2 beyond: (v01, v1, v12)
+    a0 = p; a1 = v1; a2 = q;
+  } else {               // only v0 beyond: (v01, v1, v2), (v01, v2, v02)
+    a0 = p; a1 = v1; a2 = v2;
+  }
+  b0 = p; b1 = v2; b2 = q;
+  return (beyond1 || beyond2) ? 1 : 2;
+}
+
+// Raster state of one (sub-)triangle after setup.
+struct TriSetup {
+  float n1x, n1y, n2x, n2y;  // edge normals, Renderer.cpp:215-216
+  float p0x, p0y, p2x, p2y;  // pixel positions of vertices 0 and 2
+  float z0, z1, z2;          // -1/z_eye (perspective) or NDC z (ortho)
+  int bx0, bx1, by0, by1;    // clamped pixel bbox, inclusive
+};
+
 // Triangle setup of paintTriangle (Renderer.cpp:179-236) for eye-space
 // positions.  Returns false when the triangle is rejected off-screen or
-// back-face culled.  Fills the raster record fields.
+// back-face culled.
 MR_HD inline bool setup_triangle(const float* P, bool persp, float w, float h,
-                                 f3 a, f3 b, f3 c, TriRec& r) {
+                                 f3 a, f3 b, f3 c, TriSetup& r) {
   f3 ndc0 = htransform(P, a), ndc1 = htransform(P, b), ndc2 = htransform(P, c);
   const float hw = w / 2.0f, hh = h / 2.0f;
   float p0x = (1.0f + ndc0.x) * hw, p0y = (1.0f - ndc0.y) * hh;
@@ -264,10 +317,10 @@ MR_HD inline bool setup_triangle(const float* P, bool persp, float w, float h,
   maxx = aclamp(maxx, 0.0f, w - 1);
   miny = aclamp(miny, 0.0f, h - 1);
   maxy = aclamp(maxy, 0.0f, h - 1);
-  r.bx0 = (uint16_t)(int)floorf(minx);
-  r.bx1 = (uint16_t)(int)floorf(maxx);
-  r.by0 = (uint16_t)(int)floorf(miny);
-  r.by1 = (uint16_t)(int)floorf(maxy);
+  r.bx0 = (int)floorf(minx);
+  r.bx1 = (int)floorf(maxx);
+  r.by0 = (int)floorf(miny);
+  r.by1 = (int)floorf(maxy);
   if (persp) {
     r.z0 = -1.0f / a.z;
     r.z1 = -1.0f / b.z;

@@ -1,21 +1,35 @@
 // kernels.hip -- the rasterization pipeline of minirender's Renderer::render()
 // (src/Renderer.cpp:311-381) as HIP kernels for gfx950 (wave64).
 //
-//   k_clear     counters, look-back words and tile counts of the batch
-//   k_geom      (a)+(b) vertex transform, near clip, projection, setup and
-//               cull of every input triangle; order-preserving compaction of
-//               the surviving (sub-)triangles (ballot + wave prefix + block
-//               scan + decoupled look-back), then per-tile counts
-//   k_scan      (c) exclusive scan of the tile counts (decoupled look-back)
-//   k_bin       (c) scatter of record ids into the tiles' bin lists
-//   k_tile      (d)+(e) per-tile raster with the tile's 64-bit
-//               (depth key, record id) buffer in LDS and ds_min_u64, then
-//               deferred Blinn-Phong + texture of each pixel's winner
+//   k_clear   counters, look-back words, tile counts and merge counters
+//   k_geom    (a)+(b) vertex transform, near clip, projection, setup and
+//             cull of every input triangle (Renderer.cpp:340-380, 163-236).
+//             Each wavefront compacts its survivors into its own region of
+//             the item stream (ballot + popcount prefix, no global atomic)
+//             and counts its (tile, triangle) pairs with one atomic per
+//             distinct tile
+//   k_scan    (c) exclusive scan of the tile counts (decoupled look-back);
+//             tiles with more pairs than one work item are split into parts
+//   k_bin     (c) scatter of triangle ids into the tiles' bin lists: per
+//             wavefront step, one atomic instruction for all distinct tiles
+//   k_tile    (d) per-tile raster with the tile's 64-bit
+//             (depth key, draw-order id) buffer in LDS and ds_min_u64; parts
+//             of a split tile merge through global 64-bit atomicMin.  Writes
+//             each pixel's winning id
+//   k_shade   (e) deferred Blinn-Phong + texture, one lane per pixel in image
+//             order, writing RGB8 / depth / winner / float colour / normals
 //
-// Record ids increase with draw order, so the minimum of (depth key, record
-// id) is the first-drawn triangle of minimum depth -- the winner of the
-// reference's sequential strict `z < depth` test (Renderer.cpp:267).
+// A triangle's id is 2 * (flat triangle index in the batch) + clip sub-index,
+// which increases with draw order inside a job.  The minimum of
+// (depth key, id) is therefore the first-drawn triangle of minimum depth --
+// the winner of the reference's sequential strict `z < depth`
+// (Renderer.cpp:267).  Setup is recomputed from the resident mesh wherever
+// it is needed (binning, raster, shading) instead of being stored: the same
+// fp32 operations on the same inputs give the same bits, and ALU is cheaper
+// than HBM traffic.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "common.h"
 #include "exact.h"
@@ -66,140 +80,229 @@ __device__ u64 lookback(u64* status, int idx, u64 aggregate, u64* counters) {
 }
 
 __global__ void k_clear(BatchArgs a) {
-  const int n = max(max(a.geom_chunks, a.scan_blocks),
-                    max(a.ntiles, (int)kNumCounters));
+  const int n = max(max(a.scan_blocks, a.ntiles), (int)kNumCounters);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
     if (i < kNumCounters) a.counters[i] = 0;
-    if (i < a.geom_chunks) a.geom_status[i] = 0;
     if (i < a.scan_blocks) a.scan_status[i] = 0;
-    if (i < a.ntiles) a.tile_count[i] = 0;
+    if (i < a.ntiles) {
+      a.tile_count[i] = 0;
+      a.tile_done[i] = 0;
+    }
   }
 }
 
-__device__ inline int find_draw(const DrawDesc* draws, int n, int64_t g) {
-  int lo = 0, hi = n - 1;  // last draw with tri_begin <= g
+// Last index in [lo, hi] with begin[index] <= x (begin ascending).
+template <typename T>
+__device__ inline int search_last(const T* begin, int lo, int hi, int64_t x) {
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (draws[mid].tri_begin <= g) lo = mid; else hi = mid - 1;
+    if ((int64_t)begin[mid] <= x) lo = mid; else hi = mid - 1;
   }
   return lo;
 }
 
-__device__ inline int find_job(const JobDesc* jobs, int n, int tile) {
-  int lo = 0, hi = n - 1;  // last job with tile_base <= tile
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].tile_base <= tile) lo = mid; else hi = mid - 1;
-  }
-  return lo;
+// Draw of flat triangle g in job J: the job's draw table cached in LDS when
+// it fits (ncached > 0), else searched in global memory.
+__device__ inline int lookup_draw(const int64_t* dcache, int ncached,
+                                  const BatchArgs& a, const JobDesc& J, int64_t g) {
+  if (ncached > 0) return J.draw_begin + search_last(dcache, 0, ncached - 1, g);
+  return search_last(a.draw_begin, J.draw_begin, J.draw_end - 1, g);
+}
+
+// Fill the LDS draw cache for job J; returns the number of cached entries.
+__device__ inline int fill_draw_cache(int64_t* dcache, const BatchArgs& a,
+                                      const JobDesc& J, int tid, int nt) {
+  const int nd = J.draw_end - J.draw_begin;
+  const int n = nd <= kMaxLdsDraws ? nd : 0;
+  for (int i = tid; i < n; i += nt) dcache[i] = a.draw_begin[J.draw_begin + i];
+  return n;
 }
 
 __device__ inline f3 load3(const float* p, int i) {
   return f3{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
 }
 
-// (a)+(b): one lane per input triangle (Renderer.cpp:340-380, 163-236).
-__global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
-  __shared__ int sh_chunk;
-  __shared__ uint32_t sh_wave[4];
-  __shared__ u64 sh_excl;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
-  for (;;) {
-    if (tid == 0) sh_chunk = (int)atomicAdd(&a.counters[kGeomTicket], 1ull);
-    __syncthreads();
-    const int chunk = sh_chunk;
-    if (chunk >= a.geom_chunks) break;
-    const int64_t g = (int64_t)chunk * kGeomChunk + tid;
-    TriRec r0, r1;
-    int nrec = 0;
-    bool clipped = false;
-    const JobDesc* J = nullptr;
-    if (g < a.total_tris) {
-      const int d = find_draw(a.draws, a.ndraws, g);
-      const DrawDesc& D = a.draws[d];
-      J = &a.jobs[D.job];
-      const int t = (int)(g - D.tri_begin);
-      const f3 e0 = xform(D.mv, load3(D.pos, D.idx[3 * t]));
-      const f3 e1 = xform(D.mv, load3(D.pos, D.idx[3 * t + 1]));
-      const f3 e2 = xform(D.mv, load3(D.pos, D.idx[3 * t + 2]));
-      const float zn = J->znear;
-      const bool persp = J->persp != 0;
-      const float w = (float)J->W, h = (float)J->H;
-      const bool b0 = e0.z > zn, b1 = e1.z > zn, b2 = e2.z > zn;
-      if (b0 || b1 || b2) {  // near test, Renderer.cpp:169-177
-        if (!(b0 && b1 && b2)) {
-          clipped = true;
-          Vtx v0, v1, v2, out[6];
-          v0.p = e0; v1.p = e1; v2.p = e2;
-          v0.n = v1.n = v2.n = f3{0.0f, 0.0f, 0.0f};
-          v0.uv = v1.uv = v2.uv = f2{0.0f, 0.0f};
-          const int ns = clip_triangle(zn, v0, v1, v2, out);
-#pragma unroll
-          for (int s = 0; s < 2; s++) {
-            TriRec tr;
-            if (s < ns && setup_triangle(J->P, persp, w, h, out[3 * s].p,
-                                         out[3 * s + 1].p, out[3 * s + 2].p, tr)) {
-              tr.tri_sub = ((uint32_t)t << 2) | 2u | (uint32_t)s;
-              tr.draw = (uint32_t)d;
-              tr.job = (uint32_t)D.job;
-              if (nrec == 0) r0 = tr; else r1 = tr;
-              ++nrec;
-            }
-          }
-        }
-      } else if (setup_triangle(J->P, persp, w, h, e0, e1, e2, r0)) {
-        r0.tri_sub = (uint32_t)t << 2;
-        r0.draw = (uint32_t)d;
-        r0.job = (uint32_t)D.job;
-        nrec = 1;
+// Eye-space corners of triangle t (Renderer.cpp:344-345, 364-366).
+__device__ inline void corners(const DrawDesc& D, int t, f3& a, f3& b, f3& c) {
+  const int* ip = D.idx + 3 * t;
+  a = xform(D.mv, load3(D.pos, ip[0]));
+  b = xform(D.mv, load3(D.pos, ip[1]));
+  c = xform(D.mv, load3(D.pos, ip[2]));
+}
+
+// Near test, clip and setup of triangle t (Renderer.cpp:169-236).  Returns
+// the number of surviving sub-triangles (0..2) and their sub-indices.
+__device__ inline int tri_setup(const DrawDesc& D, const JobDesc& J, int t,
+                                TriSetup& s0, int& sub0, TriSetup& s1,
+                                int& sub1, bool& clipped) {
+  f3 e0, e1, e2;
+  corners(D, t, e0, e1, e2);
+  const float zn = J.znear;
+  const bool persp = J.persp != 0;
+  const float w = (float)J.W, h = (float)J.H;
+  const bool b0 = e0.z > zn, b1 = e1.z > zn, b2 = e2.z > zn;
+  clipped = false;
+  if (b0 || b1 || b2) {
+    if (b0 && b1 && b2) return 0;
+    clipped = true;
+    f3 a0, a1, a2, c0, c1, c2;
+    const int ns = clip_positions(zn, e0, e1, e2, a0, a1, a2, c0, c1, c2);
+    int n = 0;
+    if (setup_triangle(J.P, persp, w, h, a0, a1, a2, s0)) {
+      sub0 = 0;
+      n = 1;
+    }
+    if (ns == 2) {
+      TriSetup tmp;
+      if (setup_triangle(J.P, persp, w, h, c0, c1, c2, tmp)) {
+        if (n == 0) { s0 = tmp; sub0 = 1; } else { s1 = tmp; sub1 = 1; }
+        ++n;
       }
     }
-    // Order-preserving compaction: wave ballots, block scan, look-back.
-    const u64 bal1 = __ballot(nrec >= 1), bal2 = __ballot(nrec == 2);
-    const u64 lt = (1ull << lane) - 1ull;
-    const uint32_t wpre = __popcll(bal1 & lt) + __popcll(bal2 & lt);
-    const u64 balc = __ballot(clipped);
-    if (lane == 0) {
-      sh_wave[wave] = __popcll(bal1) + __popcll(bal2);
-      if (balc) atomicAdd(&a.counters[kClipped], (u64)__popcll(balc));
-    }
-    __syncthreads();
-    uint32_t bpre = 0, btot = 0;
+    return n;
+  }
+  if (setup_triangle(J.P, persp, w, h, e0, e1, e2, s0)) {
+    sub0 = 0;
+    return 1;
+  }
+  return 0;
+}
+
+// Setup of sub-triangle `sub` of a binned triangle (which survived setup).
+__device__ inline void tri_setup_sub(const DrawDesc& D, const JobDesc& J, int t,
+                                     int sub, TriSetup& s) {
+  f3 e0, e1, e2;
+  corners(D, t, e0, e1, e2);
+  const float zn = J.znear;
+  const bool persp = J.persp != 0;
+  const float w = (float)J.W, h = (float)J.H;
+  if (e0.z > zn || e1.z > zn || e2.z > zn) {
+    f3 a0, a1, a2, c0, c1, c2;
+    clip_positions(zn, e0, e1, e2, a0, a1, a2, c0, c1, c2);
+    if (sub) setup_triangle(J.P, persp, w, h, c0, c1, c2, s);
+    else setup_triangle(J.P, persp, w, h, a0, a1, a2, s);
+    return;
+  }
+  setup_triangle(J.P, persp, w, h, e0, e1, e2, s);
+}
+
+// For every active lane's tile rectangle, visit all (lane, tile) pairs with
+// the whole wavefront (64 pairs per step, owners found by binary search of
+// the wave prefix sum) and add to ctr[tile].  Lanes with equal tiles form a
+// group; all group leaders issue their atomic in one instruction.
+// kScatter: also write the owner's id at the slot the atomic returned.
+template <bool kScatter>
+__device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id,
+                                int tbase, int tiles_x, int tx0, int tx1,
+                                int ty0, int ty1, uint32_t* ctr) {
+  const int lane = threadIdx.x & 63;
+  if (!active) { tx0 = tx1 = ty0 = ty1 = 0; }
+  const int cnt = active ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0;
+  int P = cnt;  // inclusive prefix over the wave
 #pragma unroll
-    for (int w = 0; w < 4; w++) {
-      const uint32_t v = sh_wave[w];
-      if (w < wave) bpre += v;
-      btot += v;
-    }
-    if (tid == 0) {
-      const u64 excl = lookback(a.geom_status, chunk, btot, a.counters);
-      sh_excl = excl;
-      if (chunk == a.geom_chunks - 1) a.counters[kRecords] = excl + btot;
-    }
-    __syncthreads();
-    const u64 base = sh_excl + bpre + wpre;
-    if (nrec >= 1) a.recs[base] = r0;
-    if (nrec == 2) a.recs[base + 1] = r1;
-    // Per-tile counts of the records just written.
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(P, o);
+    if (lane >= o) P += y;
+  }
+  const int C = __shfl(P, 63);
+  const int ex = P - cnt;
+  for (int j = 0; j < C; j += 64) {
+    const int q = j + lane;
+    int lo = 0, hi = 63;
 #pragma unroll
-    for (int s = 0; s < 2; s++) {
-      if (s < nrec) {
-        const TriRec& r = s == 0 ? r0 : r1;
-        uint32_t* tc = a.tile_count + J->tile_base;
-        const int tx0 = r.bx0 >> sw, tx1 = r.bx1 >> sw;
-        const int ty0 = r.by0 >> sh, ty1 = r.by1 >> sh;
-        for (int ty = ty0; ty <= ty1; ty++)
-          for (int tx = tx0; tx <= tx1; tx++)
-            atomicAdd(&tc[ty * J->tiles_x + tx], 1u);
+    for (int s = 0; s < 6; s++) {
+      const int mid = (lo + hi) >> 1;
+      if (__shfl(P, mid) > q) hi = mid; else lo = mid + 1;
+    }
+    const bool ok = q < C;
+    const int k = q - __shfl(ex, lo);
+    const int ox0 = __shfl(tx0, lo), ox1 = __shfl(tx1, lo), oy0 = __shfl(ty0, lo);
+    const int ob = __shfl(tbase, lo), otx = __shfl(tiles_x, lo);
+    const int w = max(ox1 - ox0 + 1, 1);
+    const int r = k / w;
+    const int key = ok ? ob + (oy0 + r) * otx + (ox0 + (k - r * w)) : -1;
+    u64 pend = __ballot(ok);
+    int leader = lane;
+    uint32_t rank = 0, gsize = 0;
+    while (pend) {
+      const int l = __builtin_ctzll(pend);
+      const int kk = __builtin_amdgcn_readlane(key, l);
+      const u64 grp = pend & __ballot(key == kk);
+      if ((grp >> lane) & 1ull) {
+        leader = l;
+        rank = (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
+      }
+      if (lane == l) gsize = (uint32_t)__popcll(grp);
+      pend &= ~grp;
+    }
+    uint32_t old = 0;
+    if (ok && lane == leader) old = atomicAdd(&ctr[key], gsize);
+    if (kScatter) {
+      const uint32_t oid = __shfl(id, lo);
+      old = __shfl(old, leader);
+      if (ok) {
+        const uint32_t slot = old + rank;
+        if ((u64)slot < a.bins_cap) a.bins[slot] = oid;
+        else atomicOr(&a.counters[kBinOverflow], 1ull);
       }
     }
-    __syncthreads();  // sh_chunk / sh_wave are rewritten next iteration
   }
 }
 
-// (c) exclusive scan of tile counts -> tile offsets and cursors.
+// (a)+(b): one lane per input triangle; wavefront w writes its survivors to
+// items[w * kRegion ...] and their number to region_count[w].
+__global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
+  uint32_t nclipped = 0;
+  for (int64_t base = (int64_t)blockIdx.x * 256; base < a.total_tris;
+       base += (int64_t)gridDim.x * 256) {
+    const int64_t wfirst = base + (threadIdx.x & ~63);
+    if (wfirst >= a.total_tris) continue;
+    const int64_t g = wfirst + lane;
+    TriSetup s0, s1;
+    s0.bx0 = s0.bx1 = s0.by0 = s0.by1 = 0;
+    s1.bx0 = s1.bx1 = s1.by0 = s1.by1 = 0;
+    int sub0 = 0, sub1 = 0, ns = 0;
+    bool clipped = false;
+    uint32_t job = 0;
+    int tbase = 0, tiles_x = 1;
+    if (g < a.total_tris) {
+      const int d = search_last(a.draw_begin, 0, a.ndraws - 1, g);
+      const DrawDesc& D = a.draws[d];
+      const JobDesc& J = a.jobs[D.job];
+      ns = tri_setup(D, J, (int)(g - D.tri_begin), s0, sub0, s1, sub1, clipped);
+      job = (uint32_t)D.job;
+      tbase = J.tile_base;
+      tiles_x = J.tiles_x;
+    }
+    nclipped += (uint32_t)__popcll(__ballot(clipped));
+    const u64 b1 = __ballot(ns >= 1), b2 = __ballot(ns == 2);
+    const int64_t region = wfirst >> 6;
+    if (lane == 0) a.region_count[region] = (uint32_t)(__popcll(b1) + __popcll(b2));
+    if (!b1) continue;
+    const u64 lt = (1ull << lane) - 1ull;
+    const int64_t pos = region * kRegion + __popcll(b1 & lt) + __popcll(b2 & lt);
+    const uint32_t id0 = (uint32_t)(2 * g + sub0), id1 = (uint32_t)(2 * g + sub1);
+    if (ns >= 1)
+      a.items[pos] = BinItem{id0, job, (uint16_t)s0.bx0, (uint16_t)s0.bx1,
+                             (uint16_t)s0.by0, (uint16_t)s0.by1};
+    if (ns == 2)
+      a.items[pos + 1] = BinItem{id1, job, (uint16_t)s1.bx0, (uint16_t)s1.bx1,
+                                 (uint16_t)s1.by0, (uint16_t)s1.by1};
+    wave_tile_pairs<false>(a, ns >= 1, id0, tbase, tiles_x, s0.bx0 >> sw,
+                           s0.bx1 >> sw, s0.by0 >> sh, s0.by1 >> sh, a.tile_count);
+    if (b2)
+      wave_tile_pairs<false>(a, ns == 2, id1, tbase, tiles_x, s1.bx0 >> sw,
+                             s1.bx1 >> sw, s1.by0 >> sh, s1.by1 >> sh, a.tile_count);
+  }
+  if (lane == 0 && nclipped) atomicAdd(&a.counters[kClipped], (u64)nclipped);
+}
+
+// (c) exclusive scan of (tile, triangle) pairs per tile -> offsets and
+// cursors; tiles with more than frag_size pairs get a merge slot and their
+// parts are listed for the raster.
 __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
   __shared__ int sh_blk;
   __shared__ u64 sh_w[kScanBlock / 64];
@@ -239,52 +342,214 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
   u64 run = sh_excl + wpre + (x - sum);
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    if (base + i < a.ntiles) {
+    const int t = base + i;
+    if (t < a.ntiles) {
       const uint32_t o = run > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)run;
-      a.tile_off[base + i] = o;
-      a.tile_cursor[base + i] = o;
+      a.tile_off[t] = o;
+      a.tile_cursor[t] = o;
+      uint32_t slot = kNoSlot;
+      if (v[i] > a.frag_size) {
+        const uint32_t nparts = (v[i] + a.frag_size - 1) / a.frag_size;
+        const u64 s = atomicAdd(&a.counters[kHeavy], 1ull);
+        const u64 e = atomicAdd(&a.counters[kParts], (u64)nparts);
+        if (s < a.heavy_cap && e + nparts <= a.parts_cap) {
+          slot = (uint32_t)s;
+          for (uint32_t p = 0; p < nparts; p++) a.parts[e + p] = make_uint2((uint32_t)t, p);
+        } else {
+          atomicOr(&a.counters[kHeavyOverflow], 1ull);
+        }
+      }
+      a.tile_heavy[t] = slot;
     }
     run += v[i];
   }
 }
 
-// (c) scatter record ids into the bin lists of the tiles they overlap.
+// (c) scatter triangle ids into the bin lists of the tiles they overlap;
+// also resets the merge buffers of split tiles.
 __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
-  const u64 nrec = a.counters[kRecords];
+  const int lane = threadIdx.x & 63;
+  const u64 np = (u64)a.tile_w * a.tile_h;
+  const u64 nheavy = min(a.counters[kHeavy], (u64)a.heavy_cap);
+  for (u64 i = (u64)blockIdx.x * 256 + threadIdx.x; i < nheavy * np;
+       i += (u64)gridDim.x * 256)
+    a.heavy_keys[i] = ~0ull;
   const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
-  for (u64 i = (u64)blockIdx.x * 256 + threadIdx.x; i < nrec;
-       i += (u64)gridDim.x * 256) {
-    const uint4 q = reinterpret_cast<const uint4*>(a.recs + i)[3];
-    const int bx0 = q.x & 0xFFFFu, bx1 = q.x >> 16;
-    const int by0 = q.y & 0xFFFFu, by1 = q.y >> 16;
-    const JobDesc& J = a.jobs[q.w];
-    uint32_t* cur = a.tile_cursor + J.tile_base;
-    for (int ty = by0 >> sh; ty <= (by1 >> sh); ty++)
-      for (int tx = bx0 >> sw; tx <= (bx1 >> sw); tx++) {
-        const uint32_t pos = atomicAdd(&cur[ty * J.tiles_x + tx], 1u);
-        if ((u64)pos < a.bins_cap) a.bins[pos] = (uint32_t)i;
-        else atomicOr(&a.counters[kBinOverflow], 1ull);
-      }
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  u64 nitems = 0;
+  for (int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); r < a.nregions; r += nw) {
+    const uint32_t cnt = a.region_count[r];
+    nitems += cnt;
+    for (uint32_t j = 0; j < cnt; j += 64) {
+      const bool act = j + lane < cnt;
+      BinItem it = act ? a.items[r * kRegion + j + lane] : BinItem{0, 0, 0, 0, 0, 0};
+      const JobDesc& J = a.jobs[it.job];
+      wave_tile_pairs<true>(a, act, it.id, J.tile_base, J.tiles_x, it.bx0 >> sw,
+                            it.bx1 >> sw, it.by0 >> sh, it.by1 >> sh, a.tile_cursor);
+    }
   }
+  if (lane == 0 && nitems) atomicAdd(&a.counters[kItems], nitems);
 }
 
-// Depth test of one covered-or-not pixel into the tile's LDS key buffer.
+// Depth test of one pixel into the tile's LDS key buffer.
 __device__ inline void raster_pixel(u64* keys, int p, float e1, float e2,
-                                    float z0, float z1, float z2, bool persp,
-                                    uint32_t rid) {
+                                    const TriSetup& s, bool persp, uint32_t id) {
   if (!covered(e1, e2)) return;
   float k[3];
-  const float z = depth_of(e1, e2, z0, z1, z2, persp, k);
+  const float z = depth_of(e1, e2, s.z0, s.z1, s.z2, persp, k);
   if (!(z < kClearDepth)) return;  // strict test against the clear value
-  const u64 key = ((u64)depth_key(z) << 32) | rid;
+  const u64 key = ((u64)depth_key(z) << 32) | id;
   if (key < keys[p]) atomicMin(&keys[p], key);
 }
 
-// Deferred shading of one pixel from its winning record (Renderer.cpp:271-305).
-__device__ void shade_pixel(const BatchArgs& a, const JobDesc& J, u64 key,
+// (d): one workgroup per raster work item (a tile, or a part of a split
+// tile), persistent over a work-item ticket.  Split-tile parts come first.
+template <int TW, int TH, int NT>
+__global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
+  constexpr int NP = TW * TH;
+  __shared__ u64 keys[NP];
+  __shared__ TriSetup big[NT];
+  __shared__ uint32_t bigid[NT];
+  __shared__ int64_t dcache[kMaxLdsDraws];
+  __shared__ int nbig;
+  __shared__ int sh_item;
+  __shared__ int sh_last;
+  const int tid = threadIdx.x;
+  const int64_t nparts_all = (int64_t)min(a.counters[kParts], (u64)a.parts_cap);
+  const int64_t nitems = nparts_all + a.ntiles;
+  int cached_job = -1, ncached = 0;
+  for (;;) {
+    if (tid == 0) sh_item = (int)atomicAdd(&a.counters[kTileTicket], 1ull);
+    __syncthreads();
+    const int64_t item = sh_item;
+    if (item >= nitems) break;
+    int tile, part;
+    if (item < nparts_all) {
+      const uint2 pp = a.parts[item];
+      tile = (int)pp.x;
+      part = (int)pp.y;
+    } else {
+      tile = (int)(item - nparts_all);
+      part = 0;
+    }
+    const uint32_t cnt = a.tile_count[tile];
+    const uint32_t nparts = cnt <= a.frag_size ? 1u : (cnt + a.frag_size - 1) / a.frag_size;
+    if (item >= nparts_all && nparts > 1) {  // rendered through its parts
+      __syncthreads();
+      continue;
+    }
+    const int j = search_last(a.job_tile_base, 0, a.njobs - 1, tile);
+    const JobDesc& J = a.jobs[j];
+    if (j != cached_job) {
+      ncached = fill_draw_cache(dcache, a, J, tid, NT);
+      cached_job = j;
+    }
+    const bool persp = J.persp != 0;
+    const int local = tile - J.tile_base;
+    const int tyi = local / J.tiles_x, txi = local - tyi * J.tiles_x;
+    const int x0 = txi * TW, y0 = tyi * TH;
+    for (int p = tid; p < NP; p += NT) keys[p] = ~0ull;
+    const uint32_t off = a.tile_off[tile];
+    uint32_t avail = cnt;  // entries actually written (bin-list overflow)
+    if ((u64)off >= a.bins_cap) avail = 0;
+    else if ((u64)off + cnt > a.bins_cap) avail = (uint32_t)(a.bins_cap - off);
+    const uint32_t e_begin = (uint32_t)part * a.frag_size;
+    const uint32_t e_end = min(avail, e_begin + a.frag_size);
+    __syncthreads();
+    for (uint32_t base = e_begin; base < e_end; base += NT) {
+      if (tid == 0) nbig = 0;
+      __syncthreads();
+      if (base + tid < e_end) {
+        const uint32_t id = a.bins[off + base + tid];
+        const int64_t g = id >> 1;
+        const DrawDesc& D = a.draws[lookup_draw(dcache, ncached, a, J, g)];
+        TriSetup s;
+        tri_setup_sub(D, J, (int)(g - D.tri_begin), (int)(id & 1u), s);
+        const int cx0 = max(s.bx0, x0), cx1 = min(s.bx1, x0 + TW - 1);
+        const int cy0 = max(s.by0, y0), cy1 = min(s.by1, y0 + TH - 1);
+        if ((cx1 - cx0 + 1) * (cy1 - cy0 + 1) <= kSmallArea) {
+          // One lane walks the triangle's rows inside the tile, as the
+          // reference's loops do (Renderer.cpp:238-269).
+          const float x0f = (float)s.bx0 + 0.5f;
+          const int lead = cx0 - s.bx0;
+          for (int row = cy0; row <= cy1; row++) {
+            float e1, e2;
+            edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
+                       x0f, (float)row + 0.5f, e1, e2);
+            e1 = advance(e1, s.n1x, lead);
+            e2 = advance(e2, s.n2x, lead);
+            const int prow = (row - y0) * TW - x0;
+            for (int c = cx0; c <= cx1; c++) {
+              raster_pixel(keys, prow + c, e1, e2, s, persp, id);
+              e1 = e1 + s.n1x;
+              e2 = e2 + s.n2x;
+            }
+          }
+        } else {
+          const int slot = atomicAdd(&nbig, 1);
+          big[slot] = s;
+          bigid[slot] = id;
+        }
+      }
+      __syncthreads();
+      const int nb = nbig;
+      for (int b = 0; b < nb; b++) {  // large triangles: one lane per pixel
+        const TriSetup& s = big[b];
+        for (int p = tid; p < NP; p += NT) {
+          const int c = x0 + (p % TW), row = y0 + (p / TW);
+          if (c < s.bx0 || c > s.bx1 || row < s.by0 || row > s.by1) continue;
+          float e1, e2;
+          edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
+                     (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
+          const int n = c - s.bx0;
+          e1 = advance(e1, s.n1x, n);
+          e2 = advance(e2, s.n2x, n);
+          raster_pixel(keys, p, e1, e2, s, persp, bigid[b]);
+        }
+      }
+      __syncthreads();
+    }
+    if (nparts == 1) {
+      for (int p = tid; p < NP; p += NT) {
+        const int c = x0 + (p % TW), row = y0 + (p / TW);
+        if (c < J.W && row < J.H)
+          J.win[(size_t)row * J.W + c] = keys[p] == ~0ull ? kNoWinner : (uint32_t)keys[p];
+      }
+    } else {
+      // Split tile: merge into the tile's global key buffer; the last part
+      // to finish writes the winners.
+      const uint32_t slot = a.tile_heavy[tile];
+      if (slot != kNoSlot) {
+        u64* gk = a.heavy_keys + (u64)slot * NP;
+        for (int p = tid; p < NP; p += NT)
+          if (keys[p] != ~0ull) atomicMin(&gk[p], keys[p]);
+        __threadfence();
+        __syncthreads();
+        if (tid == 0)
+          sh_last = __hip_atomic_fetch_add(&a.tile_done[tile], 1u, __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_AGENT) == nparts - 1;
+        __syncthreads();
+        if (sh_last) {
+          __threadfence();
+          for (int p = tid; p < NP; p += NT) {
+            const int c = x0 + (p % TW), row = y0 + (p / TW);
+            const u64 k = __hip_atomic_load(&gk[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (c < J.W && row < J.H)
+              J.win[(size_t)row * J.W + c] = k == ~0ull ? kNoWinner : (uint32_t)k;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Deferred shading of one pixel from its winner (Renderer.cpp:271-305).
+__device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
+                            const int64_t* dcache, int ncached, uint32_t id,
                             int c, int row) {
   const size_t pix = (size_t)row * (size_t)J.W + (size_t)c;
-  if (key == ~0ull) {  // clear(), Renderer.cpp:113-119
+  if (id == kNoWinner) {  // clear(), Renderer.cpp:113-119
     if (J.rgb8) {
       J.rgb8[pix * 3 + 0] = quantize(J.bg[0]);
       J.rgb8[pix * 3 + 1] = quantize(J.bg[1]);
@@ -304,21 +569,26 @@ __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J, u64 key,
     }
     return;
   }
-  const uint32_t rid = (uint32_t)key;
-  const TriRec r = a.recs[rid];
-  const DrawDesc& D = a.draws[r.draw];
+  const int64_t g = id >> 1;
+  const int sub = (int)(id & 1u);
+  const DrawDesc& D = a.draws[lookup_draw(dcache, ncached, a, J, g)];
+  const int tri = (int)(g - D.tri_begin);
+  TriSetup s;
+  tri_setup_sub(D, J, tri, sub, s);
   float e1, e2;
-  edge_start(r.n1x, r.n1y, r.n2x, r.n2y, r.p0x, r.p0y, r.p2x, r.p2y,
-             (float)r.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
-  const int n = c - (int)r.bx0;
-  e1 = advance(e1, r.n1x, n);
-  e2 = advance(e2, r.n2x, n);
+  edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
+             (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
+  const int n = c - s.bx0;
+  e1 = advance(e1, s.n1x, n);
+  e2 = advance(e2, s.n2x, n);
   float k[3];
-  const float z = depth_of(e1, e2, r.z0, r.z1, r.z2, J.persp != 0, k);
+  const float z = depth_of(e1, e2, s.z0, s.z1, s.z2, J.persp != 0, k);
+  if (J.depth) J.depth[pix] = z;
+  if (J.winner) J.winner[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
+  if (!J.rgb8 && !J.rgb && !(J.normals && J.save_normals)) return;
 
   // The triangle's eye-space vertices, recomputed as paintMesh does
   // (Renderer.cpp:344-379), and re-clipped for a clipped sub-triangle.
-  const int tri = (int)(r.tri_sub >> 2);
   Vtx v0, v1, v2;
   {
     const int* ip = D.idx + 3 * tri;
@@ -338,11 +608,11 @@ __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J, u64 key,
       v0.uv = v1.uv = v2.uv = f2{0.0f, 0.0f};
     }
   }
-  if (r.tri_sub & 2u) {
-    Vtx out[6];
-    clip_triangle(J.znear, v0, v1, v2, out);
-    if (r.tri_sub & 1u) { v0 = out[3]; v1 = out[4]; v2 = out[5]; }
-    else { v0 = out[0]; v1 = out[1]; v2 = out[2]; }
+  if (v0.p.z > J.znear || v1.p.z > J.znear || v2.p.z > J.znear) {
+    Vtx a0, a1, a2, b0, b1, b2;
+    clip_triangle(J.znear, v0, v1, v2, a0, a1, a2, b0, b1, b2);
+    if (sub) { v0 = b0; v1 = b1; v2 = b2; }
+    else { v0 = a0; v1 = a1; v2 = a2; }
   }
 
   f3 color = f3{D.diffuse[0], D.diffuse[1], D.diffuse[2]};
@@ -353,13 +623,13 @@ __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J, u64 key,
                   tex_index(u, D.tex_cols));
   }
   f3 value = f3{D.emissive[0], D.emissive[1], D.emissive[2]};
-  f3 normal = f3{0.0f, 0.0f, 1.0f};
   if (J.lighting) {  // :279-304, FAST_LIGHT
     const f3 position = add3(add3(scale3(k[0], v0.p), scale3(k[1], v1.p)),
                              scale3(k[2], v2.p));
     const f3 L = f3{J.light[0], J.light[1], J.light[2]};
     const f3 lightdir = J.light_is_point ? norm3(sub3(L, position)) : L;
-    normal = add3(add3(scale3(k[0], v0.n), scale3(k[1], v1.n)), scale3(k[2], v2.n));
+    const f3 normal = add3(add3(scale3(k[0], v0.n), scale3(k[1], v1.n)),
+                           scale3(k[2], v2.n));
     const float diff = amax(0.0f, dot3(normal, lightdir)) / len3(normal) + J.ambient;
     value = add3(value, scale3(diff, color));
     if (D.shininess != 0.0f) {
@@ -384,8 +654,6 @@ __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J, u64 key,
     J.rgb8[pix * 3 + 1] = quantize(value.y);
     J.rgb8[pix * 3 + 2] = quantize(value.z);
   }
-  if (J.depth) J.depth[pix] = z;
-  if (J.winner) J.winner[pix] = D.order_base + 2u * (uint32_t)tri + (r.tri_sub & 1u);
   if (J.rgb) {
     J.rgb[pix * 3 + 0] = value.x;
     J.rgb[pix * 3 + 1] = value.y;
@@ -393,103 +661,41 @@ __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J, u64 key,
   }
 }
 
-// (d)+(e): one workgroup per tile, persistent over a tile ticket.
-template <int TW, int TH, int NT>
-__global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
-  constexpr int NP = TW * TH;
-  __shared__ u64 keys[NP];
-  __shared__ TriRec big[NT];
-  __shared__ uint32_t bigid[NT];
-  __shared__ int nbig;
-  __shared__ int sh_tile;
-  const int tid = threadIdx.x;
-  for (;;) {
-    if (tid == 0) sh_tile = (int)atomicAdd(&a.counters[kTileTicket], 1ull);
-    __syncthreads();
-    const int tile = sh_tile;
-    if (tile >= a.ntiles) break;
-    const JobDesc& J = a.jobs[find_job(a.jobs, a.njobs, tile)];
-    const bool persp = J.persp != 0;
-    const int local = tile - J.tile_base;
-    const int tyi = local / J.tiles_x, txi = local - tyi * J.tiles_x;
-    const int x0 = txi * TW, y0 = tyi * TH;
-    for (int p = tid; p < NP; p += NT) keys[p] = ~0ull;
-    const uint32_t off = a.tile_off[tile];
-    uint32_t cnt = a.tile_count[tile];
-    if ((u64)off >= a.bins_cap) cnt = 0;
-    else if ((u64)off + cnt > a.bins_cap) cnt = (uint32_t)(a.bins_cap - off);
-    __syncthreads();
-    for (uint32_t base = 0; base < cnt; base += NT) {
-      if (tid == 0) nbig = 0;
+// (e): one lane per pixel, kShadeBlock consecutive pixels of one job per
+// workgroup iteration.
+__global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
+  __shared__ int64_t dcache[kMaxLdsDraws];
+  int cached_job = -1, ncached = 0;
+  for (int b = blockIdx.x; b < a.shade_blocks; b += gridDim.x) {
+    const int j = search_last(a.job_shade_base, 0, a.njobs - 1, b);
+    const JobDesc& J = a.jobs[j];
+    if (j != cached_job) {
       __syncthreads();
-      if (base + tid < cnt) {
-        const uint32_t rid = a.bins[off + base + tid];
-        const TriRec r = a.recs[rid];
-        const int cx0 = max((int)r.bx0, x0), cx1 = min((int)r.bx1, x0 + TW - 1);
-        const int cy0 = max((int)r.by0, y0), cy1 = min((int)r.by1, y0 + TH - 1);
-        if ((cx1 - cx0 + 1) * (cy1 - cy0 + 1) <= kSmallArea) {
-          // One lane walks the triangle's rows inside the tile, as the
-          // reference's loops do (Renderer.cpp:238-269).
-          const float x0f = (float)r.bx0 + 0.5f;
-          const int lead = cx0 - (int)r.bx0;
-          for (int row = cy0; row <= cy1; row++) {
-            float e1, e2;
-            edge_start(r.n1x, r.n1y, r.n2x, r.n2y, r.p0x, r.p0y, r.p2x, r.p2y,
-                       x0f, (float)row + 0.5f, e1, e2);
-            e1 = advance(e1, r.n1x, lead);
-            e2 = advance(e2, r.n2x, lead);
-            const int prow = (row - y0) * TW - x0;
-            for (int c = cx0; c <= cx1; c++) {
-              raster_pixel(keys, prow + c, e1, e2, r.z0, r.z1, r.z2, persp, rid);
-              e1 = e1 + r.n1x;
-              e2 = e2 + r.n2x;
-            }
-          }
-        } else {
-          const int s = atomicAdd(&nbig, 1);
-          big[s] = r;
-          bigid[s] = rid;
-        }
-      }
-      __syncthreads();
-      const int nb = nbig;
-      for (int b = 0; b < nb; b++) {  // large triangles: one lane per pixel
-        const TriRec& r = big[b];
-        for (int p = tid; p < NP; p += NT) {
-          const int c = x0 + (p % TW), row = y0 + (p / TW);
-          if (c < (int)r.bx0 || c > (int)r.bx1 || row < (int)r.by0 || row > (int)r.by1)
-            continue;
-          float e1, e2;
-          edge_start(r.n1x, r.n1y, r.n2x, r.n2y, r.p0x, r.p0y, r.p2x, r.p2y,
-                     (float)r.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
-          const int n = c - (int)r.bx0;
-          e1 = advance(e1, r.n1x, n);
-          e2 = advance(e2, r.n2x, n);
-          raster_pixel(keys, p, e1, e2, r.z0, r.z1, r.z2, persp, bigid[b]);
-        }
-      }
+      ncached = fill_draw_cache(dcache, a, J, threadIdx.x, kShadeBlock);
+      cached_job = j;
       __syncthreads();
     }
-    for (int p = tid; p < NP; p += NT) {
-      const int c = x0 + (p % TW), row = y0 + (p / TW);
-      if (c < J.W && row < J.H) shade_pixel(a, J, keys[p], c, row);
+    const int64_t pix = (int64_t)(b - J.shade_base) * kShadeBlock + threadIdx.x;
+    if (pix < (int64_t)J.W * J.H) {
+      const int row = (int)(pix / J.W), c = (int)(pix - (int64_t)row * J.W);
+      shade_pixel(a, J, dcache, ncached, J.win[pix], c, row);
     }
-    __syncthreads();
   }
 }
 
 // ---- launchers -------------------------------------------------------------
 
 hipError_t launch_clear(const BatchArgs& a, hipStream_t s) {
-  const int n = max(max(a.geom_chunks, a.scan_blocks), max(a.ntiles, (int)kNumCounters));
-  const int grid = min((n + 255) / 256, 4096);
+  const int n = std::max(std::max(a.scan_blocks, a.ntiles), (int)kNumCounters);
+  const int grid = std::min((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_clear, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_geometry(const BatchArgs& a, hipStream_t s) {
-  if (a.geom_chunks == 0) return hipSuccess;
-  const int grid = min(a.geom_chunks, a.num_cus * 8);
+  if (a.total_tris == 0) return hipSuccess;
+  const int64_t blocks = (a.total_tris + 255) / 256;
+  const int grid = (int)std::min<int64_t>(blocks, (int64_t)a.num_cus * 8);
   hipLaunchKernelGGL(k_geom, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
@@ -500,10 +706,8 @@ hipError_t launch_scan(const BatchArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_bin(const BatchArgs& a, hipStream_t s) {
-  const int64_t maxrec = 2 * a.total_tris;
-  if (maxrec == 0) return hipSuccess;
-  const int64_t blocks = (maxrec + 255) / 256;
-  const int grid = (int)(blocks < (int64_t)a.num_cus * 16 ? blocks : (int64_t)a.num_cus * 16);
+  const int64_t blocks = std::max<int64_t>((a.nregions + 3) / 4, 1);
+  const int grid = (int)std::min<int64_t>(blocks, (int64_t)a.num_cus * 8);
   hipLaunchKernelGGL(k_bin, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
@@ -513,7 +717,7 @@ bool tile_supported(int tw, int th) {
 }
 
 hipError_t launch_tile(const BatchArgs& a, hipStream_t s) {
-  const int grid = min(a.ntiles, a.num_cus * 8);
+  const int grid = std::min(a.ntiles + 64, a.num_cus * 8);
   if (a.tile_w == 8 && a.tile_h == 8)
     hipLaunchKernelGGL((k_tile<8, 8, 64>), dim3(grid), dim3(64), 0, s, a);
   else if (a.tile_w == 16 && a.tile_h == 16)
@@ -522,6 +726,12 @@ hipError_t launch_tile(const BatchArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((k_tile<32, 32, 256>), dim3(grid), dim3(256), 0, s, a);
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_shade(const BatchArgs& a, hipStream_t s) {
+  const int grid = std::min(a.shade_blocks, a.num_cus * 16);
+  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kShadeBlock), 0, s, a);
   return hipGetLastError();
 }
 
