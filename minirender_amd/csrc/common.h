@@ -89,6 +89,21 @@ enum Counter : int {
   kNumCounters = 16
 };
 
+// Workgroup index b in [0, n) -> work index, a bijection under which each of
+// the 8 XCDs (which receive workgroups round-robin) gets one contiguous range
+// of work: neighbouring tiles share vertices, and each XCD has its own L2.
+// XCD x = b % 8 owns q + (x < n % 8) items starting at x * q + min(x, n % 8).
+MR_HD inline int64_t xcd_remap(int64_t b, int64_t n) {
+  constexpr int64_t kXcds = 8;
+  const int64_t q = n / kXcds, rem = n % kXcds;
+  const int64_t x = b % kXcds, k = b / kXcds;
+  return x * q + (x < rem ? x : rem) + k;
+}
+
+// Per-tile counters are spread one per 64-byte line: the atomics of the
+// binner then spread over all memory channels instead of a few.
+constexpr int kCtrStride = 16;
+
 constexpr int kRegion = 128;       // item slots per geometry wavefront (64 x 2)
 constexpr int kShadeBlock = 256;   // pixels per deferred-shading workgroup
 constexpr int kMaxLdsDraws = 256;  // draw-table entries cached in LDS

@@ -200,9 +200,9 @@ static int ensure_scratch(mr_ctx* c, const BatchPlan& p, uint64_t bins_cap,
   int rc;
   if ((rc = dev_alloc(c, s.items, (size_t)std::max<int64_t>(nregions, 1) * kRegion * sizeof(BinItem), true))) return rc;
   if ((rc = dev_alloc(c, s.region_count, (size_t)std::max<int64_t>(nregions, 1) * 4, true))) return rc;
-  if ((rc = dev_alloc(c, s.tile_count, nt, true))) return rc;
+  if ((rc = dev_alloc(c, s.tile_count, nt * kCtrStride, true))) return rc;
   if ((rc = dev_alloc(c, s.tile_off, nt, true))) return rc;
-  if ((rc = dev_alloc(c, s.tile_cursor, nt, true))) return rc;
+  if ((rc = dev_alloc(c, s.tile_cursor, nt * kCtrStride, true))) return rc;
   if ((rc = dev_alloc(c, s.tile_heavy, nt, true))) return rc;
   if ((rc = dev_alloc(c, s.tile_done, nt, true))) return rc;
   if ((rc = dev_alloc(c, s.parts, parts * 8, true))) return rc;
@@ -210,6 +210,7 @@ static int ensure_scratch(mr_ctx* c, const BatchPlan& p, uint64_t bins_cap,
   if ((rc = dev_alloc(c, s.heavy_keys, (size_t)heavy_slots * npt, true))) return rc;
   if ((rc = dev_alloc(c, s.scan_status, (size_t)std::max(scb, 1) * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.counters, kNumCounters * 8, true))) return rc;
+  if ((rc = dev_alloc(c, s.block_stats, 2 * kStatSlots * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.draws, nd * sizeof(DrawDesc), true))) return rc;
   if ((rc = dev_alloc(c, s.draw_begin, nd * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.jobs, p.jobs.size() * sizeof(JobDesc), true))) return rc;
@@ -245,6 +246,7 @@ BatchArgs make_args(mr_ctx* c, const BatchPlan& p) {
   a.frag_size = c->frag_size;
   a.scan_status = (unsigned long long*)s.scan_status.p;
   a.counters = (unsigned long long*)s.counters.p;
+  a.block_stats = (unsigned long long*)s.block_stats.p;
   a.draws = (const DrawDesc*)s.draws.p;
   a.draw_begin = (const int64_t*)s.draw_begin.p;
   a.ndraws = (int32_t)p.draws.size();
@@ -334,9 +336,14 @@ int run_batch(mr_ctx* c, BatchPlan& p) {
     }
   }
   c->heavy_slots = std::max<uint32_t>(c->heavy_slots, (uint32_t)hc[kHeavy]);
+  {
+    const BatchArgs a = make_args(c, p);
+    c->last_geom_grid = p.total_tris ? geom_grid(a) : 0;
+    c->last_bin_grid = bin_grid(a);
+  }
   c->stats.triangles_in = p.total_tris;
-  c->stats.triangles_clipped = (int64_t)hc[kClipped];
-  c->stats.records = (int64_t)hc[kItems];
+  c->stats.triangles_clipped = -1;  // summed from block_stats on request
+  c->stats.records = -1;
   c->stats.bin_entries = (int64_t)hc[kBinEntries];
   c->stats.tiles = p.ntiles;
   if (c->profiling) {
@@ -458,7 +465,7 @@ int mr_ctx_destroy(mr_ctx* c) {
   Scratch& s = c->sc;
   DevBuf* bufs[] = {&s.items, &s.region_count, &s.tile_count, &s.tile_off,
                     &s.tile_cursor, &s.tile_heavy, &s.tile_done, &s.parts, &s.bins,
-                    &s.heavy_keys, &s.scan_status, &s.counters, &s.draws,
+                    &s.heavy_keys, &s.scan_status, &s.counters, &s.block_stats, &s.draws,
                     &s.draw_begin, &s.jobs, &s.job_bases, &s.win, &s.outputs};
   for (DevBuf* b : bufs) dev_free(c, *b, true);
   host_free(s.h_draws);
@@ -507,6 +514,18 @@ int mr_ctx_set_profiling(mr_ctx* c, int32_t on) {
 
 int mr_ctx_get_stats(mr_ctx* c, mr_stats* st) {
   if (!c || !st) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (c->stats.records < 0 && c->sc.block_stats.p) {
+    // Sum the per-workgroup statistics of the last batch.
+    std::vector<unsigned long long> h(2 * kStatSlots);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(h.data(), c->sc.block_stats.p, h.size() * 8, hipMemcpyDeviceToHost));
+    int64_t clipped = 0, records = 0;
+    for (int i = 0; i < c->last_geom_grid; i++) clipped += (int64_t)h[i];
+    for (int i = 0; i < c->last_bin_grid; i++) records += (int64_t)h[kStatSlots + i];
+    c->stats.triangles_clipped = clipped;
+    c->stats.records = records;
+  }
   *st = c->stats;
   return MR_OK;
 }

@@ -41,8 +41,8 @@ struct TexRec {
 // Scratch of one batch in flight.
 struct Scratch {
   DevBuf items, region_count, tile_count, tile_off, tile_cursor, tile_heavy,
-      tile_done, parts, bins, heavy_keys, scan_status, counters, draws,
-      draw_begin, jobs, job_bases, win, outputs;
+      tile_done, parts, bins, heavy_keys, scan_status, counters, block_stats,
+      draws, draw_begin, jobs, job_bases, win, outputs;
   HostBuf h_draws, h_draw_begin, h_jobs, h_job_bases, h_counters;
 };
 
@@ -76,6 +76,8 @@ struct mr_ctx {
   uint32_t heavy_slots = 0;   // merge buffers of split tiles (grown on demand)
   bool profiling = false;
   mr_stats stats{};
+  int32_t last_geom_grid = 0;  // grids of the last batch, to sum block_stats
+  int32_t last_bin_grid = 0;
   hipEvent_t ev[7] = {};
   int32_t num_cus = 256;
   std::recursive_mutex mu;

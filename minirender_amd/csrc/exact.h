@@ -147,9 +147,10 @@ MR_HD inline float ff_add(float s, float d, int n) {
   return s;
 }
 
-// s after n steps of s = s + d: serial for short runs, ff_add otherwise.
+// s after n steps of s = s + d: serial for short runs (one add each), ff_add
+// (a few hundred instructions of fp64 bookkeeping) for long ones.
 MR_HD inline float advance(float s, float d, int n) {
-  if (n <= 24) {
+  if (n <= 256) {
     for (int i = 0; i < n; i++) s = s + d;
     return s;
   }

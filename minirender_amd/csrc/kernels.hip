@@ -42,7 +42,6 @@ using u64 = unsigned long long;
 constexpr u64 kFlagAgg = 1ull << 62;
 constexpr u64 kFlagPre = 2ull << 62;
 constexpr u64 kValMask = (1ull << 62) - 1;
-constexpr int kSmallArea = 24;  // pixels of a tile walked by one lane
 
 // Decoupled look-back (single thread): publish this block's aggregate, sum
 // predecessors back to the first inclusive prefix, publish ours.  Block ids
@@ -86,7 +85,7 @@ __global__ void k_clear(BatchArgs a) {
     if (i < kNumCounters) a.counters[i] = 0;
     if (i < a.scan_blocks) a.scan_status[i] = 0;
     if (i < a.ntiles) {
-      a.tile_count[i] = 0;
+      a.tile_count[(size_t)i * kCtrStride] = 0;
       a.tile_done[i] = 0;
     }
   }
@@ -189,7 +188,7 @@ __device__ inline void tri_setup_sub(const DrawDesc& D, const JobDesc& J, int t,
 
 // For every active lane's tile rectangle, visit all (lane, tile) pairs with
 // the whole wavefront (64 pairs per step, owners found by binary search of
-// the wave prefix sum) and add to ctr[tile].  Lanes with equal tiles form a
+// the wave prefix sum) and add to ctr[tile * kCtrStride].  Lanes with equal tiles form a
 // group; all group leaders issue their atomic in one instruction.
 // kScatter: also write the owner's id at the slot the atomic returned.
 template <bool kScatter>
@@ -222,10 +221,14 @@ __device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id,
     const int w = max(ox1 - ox0 + 1, 1);
     const int r = k / w;
     const int key = ok ? ob + (oy0 + r) * otx + (ox0 + (k - r * w)) : -1;
+    // Group equal keys (at most kMaxGroups groups: steps of a wide triangle
+    // have ~64 distinct tiles, where grouping only costs); ungrouped lanes
+    // do their own atomic.
     u64 pend = __ballot(ok);
     int leader = lane;
-    uint32_t rank = 0, gsize = 0;
-    while (pend) {
+    uint32_t rank = 0, gsize = 1;
+    constexpr int kMaxGroups = 8;
+    for (int it = 0; it < kMaxGroups && pend; it++) {
       const int l = __builtin_ctzll(pend);
       const int kk = __builtin_amdgcn_readlane(key, l);
       const u64 grp = pend & __ballot(key == kk);
@@ -237,7 +240,7 @@ __device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id,
       pend &= ~grp;
     }
     uint32_t old = 0;
-    if (ok && lane == leader) old = atomicAdd(&ctr[key], gsize);
+    if (ok && lane == leader) old = atomicAdd(&ctr[(size_t)key * kCtrStride], gsize);
     if (kScatter) {
       const uint32_t oid = __shfl(id, lo);
       old = __shfl(old, leader);
@@ -297,7 +300,12 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
       wave_tile_pairs<false>(a, ns == 2, id1, tbase, tiles_x, s1.bx0 >> sw,
                              s1.bx1 >> sw, s1.by0 >> sh, s1.by1 >> sh, a.tile_count);
   }
-  if (lane == 0 && nclipped) atomicAdd(&a.counters[kClipped], (u64)nclipped);
+  __shared__ uint32_t sh_clipped;
+  if (threadIdx.x == 0) sh_clipped = 0;
+  __syncthreads();
+  if (lane == 0 && nclipped) atomicAdd(&sh_clipped, nclipped);
+  __syncthreads();
+  if (threadIdx.x == 0 && blockIdx.x < kStatSlots) a.block_stats[blockIdx.x] = sh_clipped;
 }
 
 // (c) exclusive scan of (tile, triangle) pairs per tile -> offsets and
@@ -316,7 +324,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
   u64 sum = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    v[i] = (base + i < a.ntiles) ? a.tile_count[base + i] : 0u;
+    v[i] = (base + i < a.ntiles) ? a.tile_count[(size_t)(base + i) * kCtrStride] : 0u;
     sum += v[i];
   }
   u64 x = sum;  // wave inclusive scan
@@ -346,7 +354,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
     if (t < a.ntiles) {
       const uint32_t o = run > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)run;
       a.tile_off[t] = o;
-      a.tile_cursor[t] = o;
+      a.tile_cursor[(size_t)t * kCtrStride] = o;
       uint32_t slot = kNoSlot;
       if (v[i] > a.frag_size) {
         const uint32_t nparts = (v[i] + a.frag_size - 1) / a.frag_size;
@@ -388,7 +396,13 @@ __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
                             it.bx1 >> sw, it.by0 >> sh, it.by1 >> sh, a.tile_cursor);
     }
   }
-  if (lane == 0 && nitems) atomicAdd(&a.counters[kItems], nitems);
+  __shared__ unsigned long long sh_items;
+  if (threadIdx.x == 0) sh_items = 0;
+  __syncthreads();
+  if (lane == 0 && nitems) atomicAdd(&sh_items, nitems);
+  __syncthreads();
+  if (threadIdx.x == 0 && blockIdx.x < kStatSlots)
+    a.block_stats[kStatSlots + blockIdx.x] = sh_items;
 }
 
 // Depth test of one pixel into the tile's LDS key buffer.
@@ -402,48 +416,40 @@ __device__ inline void raster_pixel(u64* keys, int p, float e1, float e2,
   if (key < keys[p]) atomicMin(&keys[p], key);
 }
 
-// (d): one workgroup per raster work item (a tile, or a part of a split
-// tile), persistent over a work-item ticket.  Split-tile parts come first.
+// (d): one workgroup per raster work item: first the parts of split tiles
+// (listed by the scan), then every tile.  The grid has parts_cap + ntiles
+// workgroups; workgroups past the actual number of items exit at once.
 template <int TW, int TH, int NT>
 __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
   constexpr int NP = TW * TH;
   __shared__ u64 keys[NP];
-  __shared__ TriSetup big[NT];
-  __shared__ uint32_t bigid[NT];
+  __shared__ TriSetup setups[NT];
+  __shared__ uint32_t setup_id[NT];
+  __shared__ int row_start[NT];
+  __shared__ int wave_rows[NT / 64];
   __shared__ int64_t dcache[kMaxLdsDraws];
-  __shared__ int nbig;
-  __shared__ int sh_item;
   __shared__ int sh_last;
   const int tid = threadIdx.x;
   const int64_t nparts_all = (int64_t)min(a.counters[kParts], (u64)a.parts_cap);
-  const int64_t nitems = nparts_all + a.ntiles;
-  int cached_job = -1, ncached = 0;
-  for (;;) {
-    if (tid == 0) sh_item = (int)atomicAdd(&a.counters[kTileTicket], 1ull);
-    __syncthreads();
-    const int64_t item = sh_item;
-    if (item >= nitems) break;
-    int tile, part;
-    if (item < nparts_all) {
-      const uint2 pp = a.parts[item];
-      tile = (int)pp.x;
-      part = (int)pp.y;
-    } else {
-      tile = (int)(item - nparts_all);
-      part = 0;
-    }
-    const uint32_t cnt = a.tile_count[tile];
+  const int64_t b = blockIdx.x;
+  int tile, part;
+  if (b < nparts_all) {
+    const uint2 pp = a.parts[b];
+    tile = (int)pp.x;
+    part = (int)pp.y;
+  } else {
+    const int64_t t = b - nparts_all;
+    if (t >= a.ntiles) return;
+    tile = (int)xcd_remap(t, a.ntiles);
+    part = 0;
+  }
+  {
+    const uint32_t cnt = a.tile_count[(size_t)tile * kCtrStride];
     const uint32_t nparts = cnt <= a.frag_size ? 1u : (cnt + a.frag_size - 1) / a.frag_size;
-    if (item >= nparts_all && nparts > 1) {  // rendered through its parts
-      __syncthreads();
-      continue;
-    }
+    if (b >= nparts_all && nparts > 1) return;  // rendered through its parts
     const int j = search_last(a.job_tile_base, 0, a.njobs - 1, tile);
     const JobDesc& J = a.jobs[j];
-    if (j != cached_job) {
-      ncached = fill_draw_cache(dcache, a, J, tid, NT);
-      cached_job = j;
-    }
+    const int ncached = fill_draw_cache(dcache, a, J, tid, NT);
     const bool persp = J.persp != 0;
     const int local = tile - J.tile_base;
     const int tyi = local / J.tiles_x, txi = local - tyi * J.tiles_x;
@@ -457,54 +463,54 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
     const uint32_t e_end = min(avail, e_begin + a.frag_size);
     __syncthreads();
     for (uint32_t base = e_begin; base < e_end; base += NT) {
-      if (tid == 0) nbig = 0;
-      __syncthreads();
+      // One lane per entry: setup, and the tile rows it covers.
+      int nrows = 0;
       if (base + tid < e_end) {
         const uint32_t id = a.bins[off + base + tid];
         const int64_t g = id >> 1;
         const DrawDesc& D = a.draws[lookup_draw(dcache, ncached, a, J, g)];
         TriSetup s;
         tri_setup_sub(D, J, (int)(g - D.tri_begin), (int)(id & 1u), s);
-        const int cx0 = max(s.bx0, x0), cx1 = min(s.bx1, x0 + TW - 1);
-        const int cy0 = max(s.by0, y0), cy1 = min(s.by1, y0 + TH - 1);
-        if ((cx1 - cx0 + 1) * (cy1 - cy0 + 1) <= kSmallArea) {
-          // One lane walks the triangle's rows inside the tile, as the
-          // reference's loops do (Renderer.cpp:238-269).
-          const float x0f = (float)s.bx0 + 0.5f;
-          const int lead = cx0 - s.bx0;
-          for (int row = cy0; row <= cy1; row++) {
-            float e1, e2;
-            edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
-                       x0f, (float)row + 0.5f, e1, e2);
-            e1 = advance(e1, s.n1x, lead);
-            e2 = advance(e2, s.n2x, lead);
-            const int prow = (row - y0) * TW - x0;
-            for (int c = cx0; c <= cx1; c++) {
-              raster_pixel(keys, prow + c, e1, e2, s, persp, id);
-              e1 = e1 + s.n1x;
-              e2 = e2 + s.n2x;
-            }
-          }
-        } else {
-          const int slot = atomicAdd(&nbig, 1);
-          big[slot] = s;
-          bigid[slot] = id;
-        }
+        nrows = min(s.by1, y0 + TH - 1) - max(s.by0, y0) + 1;
+        setups[tid] = s;
+        setup_id[tid] = id;
       }
+      // Exclusive scan of the row counts over the workgroup.
+      int incl = nrows;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if ((tid & 63) >= o) incl += y;
+      }
+      if ((tid & 63) == 63) wave_rows[tid >> 6] = incl;
       __syncthreads();
-      const int nb = nbig;
-      for (int b = 0; b < nb; b++) {  // large triangles: one lane per pixel
-        const TriSetup& s = big[b];
-        for (int p = tid; p < NP; p += NT) {
-          const int c = x0 + (p % TW), row = y0 + (p / TW);
-          if (c < s.bx0 || c > s.bx1 || row < s.by0 || row > s.by1) continue;
-          float e1, e2;
-          edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
-                     (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
-          const int n = c - s.bx0;
-          e1 = advance(e1, s.n1x, n);
-          e2 = advance(e2, s.n2x, n);
-          raster_pixel(keys, p, e1, e2, s, persp, bigid[b]);
+      int before = 0, total = 0;
+#pragma unroll
+      for (int w = 0; w < NT / 64; w++) {
+        if (w < (tid >> 6)) before += wave_rows[w];
+        total += wave_rows[w];
+      }
+      row_start[tid] = before + incl - nrows;
+      __syncthreads();
+      // One lane per (triangle, tile row): start the row at the tile's left
+      // edge (exact advance of the serial recurrence), then step across the
+      // tile as the reference's column loop does (Renderer.cpp:238-269).
+      for (int q = tid; q < total; q += NT) {
+        const int i = search_last(row_start, 0, NT - 1, q);
+        const TriSetup& s = setups[i];
+        const int row = max(s.by0, y0) + (q - row_start[i]);
+        const int cx0 = max(s.bx0, x0), cx1 = min(s.bx1, x0 + TW - 1);
+        float e1, e2;
+        edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
+                   (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
+        e1 = advance(e1, s.n1x, cx0 - s.bx0);
+        e2 = advance(e2, s.n2x, cx0 - s.bx0);
+        const int prow = (row - y0) * TW - x0;
+        const uint32_t id = setup_id[i];
+        for (int c = cx0; c <= cx1; c++) {
+          raster_pixel(keys, prow + c, e1, e2, s, persp, id);
+          e1 = e1 + s.n1x;
+          e2 = e2 + s.n2x;
         }
       }
       __syncthreads();
@@ -540,7 +546,6 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
         }
       }
     }
-    __syncthreads();
   }
 }
 
@@ -583,9 +588,11 @@ __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
   e2 = advance(e2, s.n2x, n);
   float k[3];
   const float z = depth_of(e1, e2, s.z0, s.z1, s.z2, J.persp != 0, k);
-  if (J.depth) J.depth[pix] = z;
-  if (J.winner) J.winner[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
-  if (!J.rgb8 && !J.rgb && !(J.normals && J.save_normals)) return;
+  if (!J.rgb8 && !J.rgb && !(J.normals && J.save_normals)) {
+    if (J.depth) J.depth[pix] = z;
+    if (J.winner) J.winner[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
+    return;
+  }
 
   // The triangle's eye-space vertices, recomputed as paintMesh does
   // (Renderer.cpp:344-379), and re-clipped for a clipped sub-triangle.
@@ -659,27 +666,23 @@ __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
     J.rgb[pix * 3 + 1] = value.y;
     J.rgb[pix * 3 + 2] = value.z;
   }
+  if (J.depth) J.depth[pix] = z;
+  if (J.winner) J.winner[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
 }
 
 // (e): one lane per pixel, kShadeBlock consecutive pixels of one job per
-// workgroup iteration.
+// workgroup.
 __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
   __shared__ int64_t dcache[kMaxLdsDraws];
-  int cached_job = -1, ncached = 0;
-  for (int b = blockIdx.x; b < a.shade_blocks; b += gridDim.x) {
-    const int j = search_last(a.job_shade_base, 0, a.njobs - 1, b);
-    const JobDesc& J = a.jobs[j];
-    if (j != cached_job) {
-      __syncthreads();
-      ncached = fill_draw_cache(dcache, a, J, threadIdx.x, kShadeBlock);
-      cached_job = j;
-      __syncthreads();
-    }
-    const int64_t pix = (int64_t)(b - J.shade_base) * kShadeBlock + threadIdx.x;
-    if (pix < (int64_t)J.W * J.H) {
-      const int row = (int)(pix / J.W), c = (int)(pix - (int64_t)row * J.W);
-      shade_pixel(a, J, dcache, ncached, J.win[pix], c, row);
-    }
+  const int b = (int)xcd_remap(blockIdx.x, a.shade_blocks);
+  const int j = search_last(a.job_shade_base, 0, a.njobs - 1, b);
+  const JobDesc& J = a.jobs[j];
+  const int ncached = fill_draw_cache(dcache, a, J, threadIdx.x, kShadeBlock);
+  __syncthreads();
+  const int64_t pix = (int64_t)(b - J.shade_base) * kShadeBlock + threadIdx.x;
+  if (pix < (int64_t)J.W * J.H) {
+    const int row = (int)(pix / J.W), c = (int)(pix - (int64_t)row * J.W);
+    shade_pixel(a, J, dcache, ncached, J.win[pix], c, row);
   }
 }
 
@@ -694,9 +697,7 @@ hipError_t launch_clear(const BatchArgs& a, hipStream_t s) {
 
 hipError_t launch_geometry(const BatchArgs& a, hipStream_t s) {
   if (a.total_tris == 0) return hipSuccess;
-  const int64_t blocks = (a.total_tris + 255) / 256;
-  const int grid = (int)std::min<int64_t>(blocks, (int64_t)a.num_cus * 8);
-  hipLaunchKernelGGL(k_geom, dim3(grid), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_geom, dim3(geom_grid(a)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -706,9 +707,7 @@ hipError_t launch_scan(const BatchArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_bin(const BatchArgs& a, hipStream_t s) {
-  const int64_t blocks = std::max<int64_t>((a.nregions + 3) / 4, 1);
-  const int grid = (int)std::min<int64_t>(blocks, (int64_t)a.num_cus * 8);
-  hipLaunchKernelGGL(k_bin, dim3(grid), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_bin, dim3(bin_grid(a)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -717,7 +716,7 @@ bool tile_supported(int tw, int th) {
 }
 
 hipError_t launch_tile(const BatchArgs& a, hipStream_t s) {
-  const int grid = std::min(a.ntiles + 64, a.num_cus * 8);
+  const int grid = a.ntiles + (int)a.parts_cap;
   if (a.tile_w == 8 && a.tile_h == 8)
     hipLaunchKernelGGL((k_tile<8, 8, 64>), dim3(grid), dim3(64), 0, s, a);
   else if (a.tile_w == 16 && a.tile_h == 16)
@@ -730,7 +729,7 @@ hipError_t launch_tile(const BatchArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_shade(const BatchArgs& a, hipStream_t s) {
-  const int grid = std::min(a.shade_blocks, a.num_cus * 16);
+  const int grid = a.shade_blocks;
   hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kShadeBlock), 0, s, a);
   return hipGetLastError();
 }

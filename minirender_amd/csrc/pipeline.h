@@ -26,6 +26,10 @@ struct BatchArgs {
   uint32_t frag_size;        // max (tile, triangle) pairs per raster work item
   unsigned long long* scan_status;  // [scan_blocks] decoupled look-back words
   unsigned long long* counters;     // [kNumCounters]
+  // Per-workgroup statistics (no shared counter: same-address atomics
+  // serialize): [0, kStatSlots) clipped triangles per geometry workgroup,
+  // [kStatSlots, 2 kStatSlots) binned triangles per bin workgroup.
+  unsigned long long* block_stats;
   const DrawDesc* draws;
   const int64_t* draw_begin;  // [ndraws] tri_begin of each draw
   int32_t ndraws;
@@ -41,8 +45,23 @@ struct BatchArgs {
   int32_t num_cus;
 };
 
+constexpr int kStatSlots = 8192;  // >= the geometry and bin grids
+
 inline int32_t scan_blocks_for(int32_t tiles) {
   return (tiles + kScanBlock * kScanItems - 1) / (kScanBlock * kScanItems);
+}
+
+// Grid sizes of the grid-stride kernels (also used to sum their statistics).
+inline int32_t geom_grid(const BatchArgs& a) {
+  const int64_t blocks = (a.total_tris + 255) / 256;
+  const int64_t cap = (int64_t)a.num_cus * 8;
+  return (int32_t)(blocks < cap ? blocks : cap);
+}
+inline int32_t bin_grid(const BatchArgs& a) {
+  int64_t blocks = (a.nregions + 3) / 4;
+  if (blocks < 1) blocks = 1;
+  const int64_t cap = (int64_t)a.num_cus * 8;
+  return (int32_t)(blocks < cap ? blocks : cap);
 }
 
 hipError_t launch_clear(const BatchArgs& a, hipStream_t s);
