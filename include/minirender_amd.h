@@ -150,6 +150,56 @@ int mr_render(mr_ctx* ctx, const mr_frame* frame, const mr_outputs* out);
 int mr_render_batch(mr_ctx* ctx, const mr_frame* frames, int32_t nframes,
                     const mr_outputs* outs);
 
+/* ---- render service: continuous batching ------------------------------- */
+/* No counterpart in the reference (it renders one frame per call,
+ * Renderer.cpp:311): a request loop over the same pipeline.  Requests are
+ * queued by mr_server_submit; a worker thread admits them in arrival order
+ * into batches as execution slots and HBM budget free up, keeps up to
+ * `slots` batches in flight on separate streams (so their work overlaps on
+ * the device), and copies outputs back through pinned staging.  Each
+ * request's result is identical to an mr_render of the same frame. */
+typedef struct mr_server mr_server;
+
+typedef struct mr_server_config {
+  int32_t max_batch;     /* requests per batch (1 = plain per-request path)   */
+  int32_t slots;         /* batches in flight at once (>= 1)                  */
+  uint64_t hbm_budget;   /* bytes of framebuffer + scratch admitted at once   */
+                         /* (0 = no limit)                                     */
+  int32_t max_wait_us;   /* with a free slot, wait at most this long for more  */
+                         /* requests before launching a partial batch (0)      */
+} mr_server_config;
+
+typedef struct mr_request_info {
+  double submit_us;      /* submit time, microseconds since server start      */
+  double launch_us;      /* its batch was launched                              */
+  double done_us;        /* outputs written                                     */
+  int64_t batch_id;
+  int32_t batch_size;
+  int32_t status;        /* MR_OK or MR_ERR_*                                   */
+} mr_request_info;
+
+int mr_server_create(mr_ctx* ctx, const mr_server_config* cfg, mr_server** out);
+/* Stops accepting requests, finishes those queued, joins the worker. */
+int mr_server_destroy(mr_server* srv);
+/* Queue one frame.  The frame and its draws are copied; `out` must stay
+ * valid until the request is done.  Meshes/textures must be resident in the
+ * server's context.  Returns MR_ERR_BUDGET if the request alone exceeds the
+ * budget. */
+int mr_server_submit(mr_server* srv, const mr_frame* frame, const mr_outputs* out,
+                     uint64_t* ticket);
+/* done = 1 when the request's outputs are written. */
+int mr_server_poll(mr_server* srv, uint64_t ticket, int32_t* done);
+/* Wait for a request (timeout_ms < 0: no limit); fills info if non-NULL and
+ * forgets the ticket.  Returns the request's status or MR_ERR_TIMEOUT. */
+int mr_server_wait(mr_server* srv, uint64_t ticket, int32_t timeout_ms,
+                   mr_request_info* info);
+/* Replay an arrival trace: frame i is submitted at arrival_s[i] seconds
+ * after the call starts; returns when all are done.  latency_ms[i] = done
+ * time - scheduled arrival.  makespan_s = first arrival to last done. */
+int mr_server_run_trace(mr_server* srv, const mr_frame* frames, int32_t n,
+                        const double* arrival_s, const mr_outputs* outs,
+                        double* latency_ms, double* makespan_s);
+
 /* ---- host helpers of the reference API (no GPU needed) ----------------- */
 /* Projection matrices, row-major, Renderer.cpp:40-83. */
 int mr_projection_ortho(float l, float r, float b, float t, float n, float f,

@@ -197,3 +197,60 @@ class Context:
         oarr = (mr_outputs * len(frames))(*[outputs_struct(o) for o in outs])
         check(self.lib.mr_render_batch(self.h, farr, len(frames), oarr), "mr_render_batch")
         return outs
+
+
+class Server:
+    """Render service with continuous batching over a Context's resident
+    meshes (C-ABI mr_server_*).  max_batch=1, slots=1 is the plain
+    one-request-at-a-time GPU path."""
+
+    def __init__(self, ctx: Context, max_batch: int = 64, slots: int = 2,
+                 hbm_budget: int = 0, max_wait_us: int = 0):
+        from ._capi import mr_server_config
+        self.ctx = ctx
+        self.lib = ctx.lib
+        cfg = mr_server_config(max_batch, slots, hbm_budget, max_wait_us)
+        h = C.c_void_p()
+        check(self.lib.mr_server_create(ctx.h, C.byref(cfg), C.byref(h)), "mr_server_create")
+        self.h = h
+        self._pending = {}
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.mr_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, frame: FrameSpec, kinds: Sequence[str] = ("rgb8", "depth", "winner")) -> int:
+        f, keep = self.ctx.frame(frame)
+        out = alloc_outputs(frame, kinds)
+        o = outputs_struct(out)
+        t = C.c_uint64()
+        check(self.lib.mr_server_submit(self.h, C.byref(f), C.byref(o), C.byref(t)),
+              "mr_server_submit")
+        self._pending[t.value] = (out, keep, o)
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        from ._capi import mr_request_info
+        info = mr_request_info()
+        rc = self.lib.mr_server_wait(self.h, ticket, timeout_ms, C.byref(info))
+        out = self._pending.pop(ticket)[0]
+        check(rc, "mr_server_wait")
+        return out, info
+
+    def run_trace(self, structs, arrivals: Sequence[float], outs_structs):
+        """Replay prebuilt mr_frame / mr_outputs arrays at the given arrival
+        times (seconds); returns (latency_ms array, makespan_s)."""
+        n = len(arrivals)
+        arr = (C.c_double * n)(*arrivals)
+        lat = (C.c_double * n)()
+        mk = C.c_double()
+        check(self.lib.mr_server_run_trace(self.h, structs, n, arr, outs_structs, lat,
+                                           C.byref(mk)), "mr_server_run_trace")
+        return np.array(lat[:], dtype=np.float64), mk.value

@@ -65,15 +65,14 @@ class mr_stats(C.Structure):
 
 class mr_server_config(C.Structure):
     _fields_ = [
-        ("max_batch", C.c_int32), ("max_inflight", C.c_int32),
-        ("hbm_budget", C.c_uint64), ("use_graphs", C.c_int32),
-        ("batch_window_us", C.c_int32),
+        ("max_batch", C.c_int32), ("slots", C.c_int32),
+        ("hbm_budget", C.c_uint64), ("max_wait_us", C.c_int32),
     ]
 
 
-class mr_request_stats(C.Structure):
+class mr_request_info(C.Structure):
     _fields_ = [
-        ("arrival_us", C.c_double), ("admit_us", C.c_double),
+        ("submit_us", C.c_double), ("launch_us", C.c_double),
         ("done_us", C.c_double), ("batch_id", C.c_int64), ("batch_size", C.c_int32),
         ("status", C.c_int32),
     ]
@@ -103,6 +102,15 @@ SIGNATURES = {
     "mr_texture_release": (C.c_int, [_P, _I]),
     "mr_render": (C.c_int, [_P, C.POINTER(mr_frame), C.POINTER(mr_outputs)]),
     "mr_render_batch": (C.c_int, [_P, C.POINTER(mr_frame), _I, C.POINTER(mr_outputs)]),
+    "mr_server_create": (C.c_int, [_P, C.POINTER(mr_server_config), C.POINTER(_P)]),
+    "mr_server_destroy": (C.c_int, [_P]),
+    "mr_server_submit": (C.c_int, [_P, C.POINTER(mr_frame), C.POINTER(mr_outputs),
+                                   C.POINTER(C.c_uint64)]),
+    "mr_server_poll": (C.c_int, [_P, C.c_uint64, _IP]),
+    "mr_server_wait": (C.c_int, [_P, C.c_uint64, _I, C.POINTER(mr_request_info)]),
+    "mr_server_run_trace": (C.c_int, [_P, C.POINTER(mr_frame), _I, C.POINTER(C.c_double),
+                                      C.POINTER(mr_outputs), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]),
     "mr_projection_ortho": (C.c_int, [_F, _F, _F, _F, _F, _F, _FP]),
     "mr_projection_perspective": (C.c_int, [_F, _F, _F, _F, _F, _F, _FP]),
     "mr_projection_frustum": (C.c_int, [_F, _F, _F, _F, _FP]),

@@ -1,10 +1,11 @@
 // context.cpp -- C-ABI (include/minirender_amd.h) over the HIP pipeline.
 //
-// A context owns one device, one stream, the resident meshes and textures
+// A context owns one device, the resident meshes and textures
 // (Renderer::paintMesh's per-frame copies, Renderer.cpp:340-349, become one
-// upload) and the scratch of the batch in flight.  mr_render is
-// Renderer::render() (Renderer.cpp:311-332) for one frame; mr_render_batch
-// runs many frames through each kernel once.
+// upload) and a main execution slot (scratch + stream) for synchronous calls.
+// mr_render is Renderer::render() (Renderer.cpp:311-332) for one frame;
+// mr_render_batch runs many frames through each kernel once.  The serving
+// layer (server.cpp) drives further slots asynchronously.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,6 +23,15 @@
 #include "context.h"
 
 using namespace mr;
+
+#define HIP_TRY(x)             \
+  do {                         \
+    hipError_t e_ = (x);       \
+    if (e_ != hipSuccess) {    \
+      (void)hipGetLastError(); \
+      return MR_ERR_HIP;       \
+    }                          \
+  } while (0)
 
 namespace mr {
 
@@ -81,18 +91,38 @@ void host_free(HostBuf& b) {
   b.bytes = 0;
 }
 
-#define HIP_TRY(x)                               \
-  do {                                           \
-    hipError_t e_ = (x);                         \
-    if (e_ != hipSuccess) {                      \
-      (void)hipGetLastError();                   \
-      return MR_ERR_HIP;                         \
-    }                                            \
-  } while (0)
+int slot_init(Slot& s, hipStream_t external) {
+  if (!external) {
+    HIP_TRY(hipStreamCreateWithFlags(&s.own_stream, hipStreamNonBlocking));
+    s.stream = s.own_stream;
+  } else {
+    s.stream = external;
+  }
+  for (auto& e : s.ev) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+  return MR_OK;
+}
+
+void slot_free(mr_ctx* c, Slot& s) {
+  if (s.stream) hipStreamSynchronize(s.stream);
+  Scratch& x = s.sc;
+  DevBuf* bufs[] = {&x.items, &x.region_count, &x.tile_count, &x.tile_off,
+                    &x.tile_cursor, &x.tile_heavy, &x.tile_done, &x.parts, &x.bins,
+                    &x.heavy_keys, &x.scan_status, &x.counters, &x.block_stats,
+                    &x.draws, &x.draw_begin, &x.jobs, &x.job_bases, &x.win, &x.outputs};
+  for (DevBuf* b : bufs) dev_free(c, *b, true);
+  HostBuf* hbufs[] = {&x.h_draws, &x.h_draw_begin, &x.h_jobs, &x.h_job_bases,
+                      &x.h_counters, &x.h_outputs};
+  for (HostBuf* b : hbufs) host_free(*b);
+  for (auto& e : s.ev)
+    if (e) hipEventDestroy(e);
+  if (s.done) hipEventDestroy(s.done);
+  if (s.own_stream) hipStreamDestroy(s.own_stream);
+  s = Slot{};
+}
 
 // Describe frames as jobs + draws; sizes the batch.
-int build_batch(mr_ctx* c, const mr_frame* frames, int nframes,
-                BatchPlan& plan) {
+int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan) {
   plan.jobs.assign(nframes, JobDesc{});
   plan.draws.clear();
   plan.draw_begin.clear();
@@ -187,19 +217,19 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes,
   return MR_OK;
 }
 
-static int ensure_scratch(mr_ctx* c, const BatchPlan& p, uint64_t bins_cap,
+static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins_cap,
                           uint32_t heavy_slots) {
-  Scratch& s = c->sc;
+  Scratch& s = sl.sc;
   const int64_t T = p.total_tris;
   const int scb = scan_blocks_for((int32_t)p.ntiles);
   const size_t nt = (size_t)p.ntiles * 4;
   const size_t nd = std::max<size_t>(p.draws.size(), 1);
-  const int64_t nregions = (T + 63) / 64;
+  const int64_t nregions = std::max<int64_t>((T + 63) / 64, 1);
   const size_t npt = (size_t)c->tile_w * c->tile_h * 8;
   const size_t parts = (size_t)(bins_cap / c->frag_size) + heavy_slots + 16;
   int rc;
-  if ((rc = dev_alloc(c, s.items, (size_t)std::max<int64_t>(nregions, 1) * kRegion * sizeof(BinItem), true))) return rc;
-  if ((rc = dev_alloc(c, s.region_count, (size_t)std::max<int64_t>(nregions, 1) * 4, true))) return rc;
+  if ((rc = dev_alloc(c, s.items, (size_t)nregions * kRegion * sizeof(BinItem), true))) return rc;
+  if ((rc = dev_alloc(c, s.region_count, (size_t)nregions * 4, true))) return rc;
   if ((rc = dev_alloc(c, s.tile_count, nt * kCtrStride, true))) return rc;
   if ((rc = dev_alloc(c, s.tile_off, nt, true))) return rc;
   if ((rc = dev_alloc(c, s.tile_cursor, nt * kCtrStride, true))) return rc;
@@ -224,8 +254,8 @@ static int ensure_scratch(mr_ctx* c, const BatchPlan& p, uint64_t bins_cap,
   return MR_OK;
 }
 
-BatchArgs make_args(mr_ctx* c, const BatchPlan& p) {
-  Scratch& s = c->sc;
+static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
+  Scratch& s = sl.sc;
   BatchArgs a;
   memset(&a, 0, sizeof(a));
   a.items = (BinItem*)s.items.p;
@@ -264,148 +294,175 @@ BatchArgs make_args(mr_ctx* c, const BatchPlan& p) {
   return a;
 }
 
-// Launch all stages of a batch on the context stream (no host sync).
-int launch_batch(mr_ctx* c, const BatchArgs& a, bool profile) {
-  hipStream_t st = c->stream;
-  if (profile) HIP_TRY(hipEventRecord(c->ev[0], st));
+static int launch_stages(const BatchArgs& a, Slot& s, bool profile) {
+  hipStream_t st = s.stream;
+  if (profile) HIP_TRY(hipEventRecord(s.ev[0], st));
   HIP_TRY(launch_clear(a, st));
-  if (profile) HIP_TRY(hipEventRecord(c->ev[1], st));
+  if (profile) HIP_TRY(hipEventRecord(s.ev[1], st));
   HIP_TRY(launch_geometry(a, st));
-  if (profile) HIP_TRY(hipEventRecord(c->ev[2], st));
+  if (profile) HIP_TRY(hipEventRecord(s.ev[2], st));
   HIP_TRY(launch_scan(a, st));
-  if (profile) HIP_TRY(hipEventRecord(c->ev[3], st));
+  if (profile) HIP_TRY(hipEventRecord(s.ev[3], st));
   HIP_TRY(launch_bin(a, st));
-  if (profile) HIP_TRY(hipEventRecord(c->ev[4], st));
+  if (profile) HIP_TRY(hipEventRecord(s.ev[4], st));
   HIP_TRY(launch_tile(a, st));
-  if (profile) HIP_TRY(hipEventRecord(c->ev[5], st));
+  if (profile) HIP_TRY(hipEventRecord(s.ev[5], st));
   HIP_TRY(launch_shade(a, st));
-  if (profile) HIP_TRY(hipEventRecord(c->ev[6], st));
+  if (profile) HIP_TRY(hipEventRecord(s.ev[6], st));
   return MR_OK;
 }
 
-// Runs a batch whose jobs' output pointers are device pointers.  Handles a
-// bin-list overflow by growing the list and re-running.  Synchronous.
-int run_batch(mr_ctx* c, BatchPlan& p) {
-  uint64_t want_bins = std::max<uint64_t>(
+static uint64_t bins_wanted(mr_ctx* c, const Slot& s, const BatchPlan& p) {
+  uint64_t want = std::max<uint64_t>(
       (uint64_t)((double)p.total_tris * c->bin_factor) + (uint64_t)p.ntiles, 1u << 16);
-  if (c->sc.bins.bytes / 4 > want_bins) want_bins = c->sc.bins.bytes / 4;
+  return std::max<uint64_t>(want, s.sc.bins.bytes / 4);
+}
+
+int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
+  const uint64_t want_bins = bins_wanted(c, s, p);
   const uint32_t want_heavy = std::max<uint32_t>(
-      c->heavy_slots, (uint32_t)std::min<uint64_t>(want_bins / c->frag_size + 16, 1u << 20));
-  int rc = ensure_scratch(c, p, want_bins, want_heavy);
+      s.heavy_slots, (uint32_t)std::min<uint64_t>(want_bins / c->frag_size + 16, 1u << 20));
+  int rc = ensure_scratch(c, s, p, want_bins, want_heavy);
   if (rc) return rc;
-  Scratch& s = c->sc;
-  uint32_t* win = (uint32_t*)s.win.p;
+  Scratch& x = s.sc;
+  uint32_t* win = (uint32_t*)x.win.p;
   for (auto& J : p.jobs) {
     J.win = win;
     win += (size_t)J.W * J.H;
   }
-  memcpy(s.h_jobs.p, p.jobs.data(), p.jobs.size() * sizeof(JobDesc));
-  HIP_TRY(hipMemcpyAsync(s.jobs.p, s.h_jobs.p, p.jobs.size() * sizeof(JobDesc),
-                         hipMemcpyHostToDevice, c->stream));
-  memcpy(s.h_job_bases.p, p.job_bases.data(), p.job_bases.size() * 4);
-  HIP_TRY(hipMemcpyAsync(s.job_bases.p, s.h_job_bases.p, p.job_bases.size() * 4,
-                         hipMemcpyHostToDevice, c->stream));
+  memcpy(x.h_jobs.p, p.jobs.data(), p.jobs.size() * sizeof(JobDesc));
+  HIP_TRY(hipMemcpyAsync(x.jobs.p, x.h_jobs.p, p.jobs.size() * sizeof(JobDesc),
+                         hipMemcpyHostToDevice, s.stream));
+  memcpy(x.h_job_bases.p, p.job_bases.data(), p.job_bases.size() * 4);
+  HIP_TRY(hipMemcpyAsync(x.job_bases.p, x.h_job_bases.p, p.job_bases.size() * 4,
+                         hipMemcpyHostToDevice, s.stream));
   if (!p.draws.empty()) {
-    memcpy(s.h_draws.p, p.draws.data(), p.draws.size() * sizeof(DrawDesc));
-    memcpy(s.h_draw_begin.p, p.draw_begin.data(), p.draws.size() * 8);
-    HIP_TRY(hipMemcpyAsync(s.draws.p, s.h_draws.p, p.draws.size() * sizeof(DrawDesc),
-                           hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(s.draw_begin.p, s.h_draw_begin.p, p.draws.size() * 8,
-                           hipMemcpyHostToDevice, c->stream));
+    memcpy(x.h_draws.p, p.draws.data(), p.draws.size() * sizeof(DrawDesc));
+    memcpy(x.h_draw_begin.p, p.draw_begin.data(), p.draws.size() * 8);
+    HIP_TRY(hipMemcpyAsync(x.draws.p, x.h_draws.p, p.draws.size() * sizeof(DrawDesc),
+                           hipMemcpyHostToDevice, s.stream));
+    HIP_TRY(hipMemcpyAsync(x.draw_begin.p, x.h_draw_begin.p, p.draws.size() * 8,
+                           hipMemcpyHostToDevice, s.stream));
   }
-  unsigned long long* hc = (unsigned long long*)s.h_counters.p;
-  c->stats.retries = 0;
+  const BatchArgs a = make_args(c, s, p);
+  if ((rc = launch_stages(a, s, profile))) return rc;
+  HIP_TRY(hipMemcpyAsync(x.h_counters.p, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost,
+                         s.stream));
+  HIP_TRY(hipEventRecord(s.done, s.stream));
+  s.last_geom_grid = p.total_tris ? geom_grid(a) : 0;
+  s.last_bin_grid = bin_grid(a);
+  return MR_OK;
+}
+
+int finish_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
+  Scratch& x = s.sc;
+  const unsigned long long* hc = (const unsigned long long*)x.h_counters.p;
+  s.retries = 0;
   for (int attempt = 0;; attempt++) {
-    BatchArgs a = make_args(c, p);
-    if ((rc = launch_batch(c, a, c->profiling))) return rc;
-    HIP_TRY(hipMemcpyAsync(hc, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipEventSynchronize(s.done));
     if (hc[kError]) return MR_ERR_DEVICE;
     if (!hc[kBinOverflow] && !hc[kHeavyOverflow]) break;
     if (attempt >= 3 || hc[kBinEntries] >= 0xFFFFFFF0ull) return MR_ERR_NOMEM;
-    c->stats.retries++;
+    s.retries++;
+    int rc;
     if (hc[kBinOverflow]) {
       const uint64_t need = hc[kBinEntries] + hc[kBinEntries] / 4 + 1024;
-      if ((rc = dev_alloc(c, s.bins, (size_t)need * 4, true))) return rc;
+      if ((rc = dev_alloc(c, x.bins, (size_t)need * 4, true))) return rc;
     }
     if (hc[kHeavyOverflow]) {
-      c->heavy_slots = (uint32_t)(hc[kHeavy] + hc[kHeavy] / 4 + 16);
-      if ((rc = dev_alloc(c, s.heavy_keys,
-                          (size_t)c->heavy_slots * c->tile_w * c->tile_h * 8, true)))
+      s.heavy_slots = (uint32_t)(hc[kHeavy] + hc[kHeavy] / 4 + 16);
+      if ((rc = dev_alloc(c, x.heavy_keys,
+                          (size_t)s.heavy_slots * c->tile_w * c->tile_h * 8, true)))
         return rc;
     }
+    if ((rc = enqueue_batch(c, s, p, profile))) return rc;
   }
-  c->heavy_slots = std::max<uint32_t>(c->heavy_slots, (uint32_t)hc[kHeavy]);
-  {
-    const BatchArgs a = make_args(c, p);
-    c->last_geom_grid = p.total_tris ? geom_grid(a) : 0;
-    c->last_bin_grid = bin_grid(a);
-  }
-  c->stats.triangles_in = p.total_tris;
-  c->stats.triangles_clipped = -1;  // summed from block_stats on request
-  c->stats.records = -1;
-  c->stats.bin_entries = (int64_t)hc[kBinEntries];
-  c->stats.tiles = p.ntiles;
-  if (c->profiling) {
-    float ms = 0;
-    for (int i = 0; i < 6; i++) {
-      hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]);
-      c->stats.stage_ms[i] = ms;
+  s.heavy_slots = std::max<uint32_t>(s.heavy_slots, (uint32_t)hc[kHeavy]);
+  if (&s == &c->main) {
+    mr_stats& st = c->stats;
+    st.triangles_in = p.total_tris;
+    st.triangles_clipped = -1;  // summed from block_stats on request
+    st.records = -1;
+    st.bin_entries = (int64_t)hc[kBinEntries];
+    st.tiles = p.ntiles;
+    st.retries = s.retries;
+    if (profile) {
+      float ms = 0;
+      for (int i = 0; i < 6; i++) {
+        hipEventElapsedTime(&ms, s.ev[i], s.ev[i + 1]);
+        st.stage_ms[i] = ms;
+      }
+      hipEventElapsedTime(&ms, s.ev[0], s.ev[6]);
+      st.frame_ms = ms;
     }
-    hipEventElapsedTime(&ms, c->ev[0], c->ev[6]);
-    c->stats.frame_ms = ms;
   }
   return MR_OK;
 }
 
-// Frames with host outputs: render into device staging, copy back.
-int render_frames(mr_ctx* c, const mr_frame* frames, int n, const mr_outputs* outs) {
-  BatchPlan p;
-  int rc = build_batch(c, frames, n, p);
-  if (rc) return rc;
-  // Device staging for host outputs.
+static const size_t kOutBytes[5] = {3, 4, 4, 12, 12};  // rgb8 depth winner rgb normals
+
+int bind_outputs(mr_ctx* c, Slot& s, BatchPlan& p, const mr_outputs* outs,
+                 std::vector<size_t>& offs) {
+  const int n = (int)p.jobs.size();
   size_t total = 0;
-  std::vector<size_t> offs(n * 5, 0);
+  offs.assign((size_t)n * 5, 0);
   for (int j = 0; j < n; j++) {
     const mr_outputs& o = outs[j];
-    const size_t npix = (size_t)frames[j].width * frames[j].height;
-    const size_t sz[5] = {npix * 3, npix * 4, npix * 4, npix * 12, npix * 12};
-    void* ptrs[5] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals};
+    const size_t npix = (size_t)p.jobs[j].W * p.jobs[j].H;
+    const void* ptrs[5] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals};
     for (int k = 0; k < 5; k++) {
-      offs[j * 5 + k] = total;
-      if (ptrs[k] && !o.on_device) total += (sz[k] + 255) & ~(size_t)255;
+      offs[(size_t)j * 5 + k] = total;
+      if (ptrs[k] && !o.on_device) total += (npix * kOutBytes[k] + 255) & ~(size_t)255;
     }
   }
-  if (total && (rc = dev_alloc(c, c->sc.outputs, total, true))) return rc;
-  char* base = (char*)c->sc.outputs.p;
+  int rc;
+  if (total && (rc = dev_alloc(c, s.sc.outputs, total, true))) return rc;
+  char* base = (char*)s.sc.outputs.p;
   for (int j = 0; j < n; j++) {
     const mr_outputs& o = outs[j];
     JobDesc& J = p.jobs[j];
     if (o.on_device) {
       J.rgb8 = o.rgb8; J.depth = o.depth; J.winner = o.winner; J.rgb = o.rgb; J.normals = o.normals;
     } else {
-      J.rgb8 = o.rgb8 ? (uint8_t*)(base + offs[j * 5 + 0]) : nullptr;
-      J.depth = o.depth ? (float*)(base + offs[j * 5 + 1]) : nullptr;
-      J.winner = o.winner ? (uint32_t*)(base + offs[j * 5 + 2]) : nullptr;
-      J.rgb = o.rgb ? (float*)(base + offs[j * 5 + 3]) : nullptr;
-      J.normals = o.normals ? (float*)(base + offs[j * 5 + 4]) : nullptr;
+      J.rgb8 = o.rgb8 ? (uint8_t*)(base + offs[(size_t)j * 5 + 0]) : nullptr;
+      J.depth = o.depth ? (float*)(base + offs[(size_t)j * 5 + 1]) : nullptr;
+      J.winner = o.winner ? (uint32_t*)(base + offs[(size_t)j * 5 + 2]) : nullptr;
+      J.rgb = o.rgb ? (float*)(base + offs[(size_t)j * 5 + 3]) : nullptr;
+      J.normals = o.normals ? (float*)(base + offs[(size_t)j * 5 + 4]) : nullptr;
     }
   }
-  if ((rc = run_batch(c, p))) return rc;
+  return MR_OK;
+}
+
+int copy_outputs(mr_ctx* c, Slot& s, const BatchPlan& p, const mr_outputs* outs,
+                 const std::vector<size_t>& offs) {
+  (void)c;
   bool copied = false;
-  for (int j = 0; j < n; j++) {
+  for (size_t j = 0; j < p.jobs.size(); j++) {
     const mr_outputs& o = outs[j];
     if (o.on_device) continue;
-    const JobDesc& J = p.jobs[j];
-    const size_t npix = (size_t)J.W * J.H;
-    if (o.rgb8) { HIP_TRY(hipMemcpyAsync(o.rgb8, J.rgb8, npix * 3, hipMemcpyDeviceToHost, c->stream)); copied = true; }
-    if (o.depth) { HIP_TRY(hipMemcpyAsync(o.depth, J.depth, npix * 4, hipMemcpyDeviceToHost, c->stream)); copied = true; }
-    if (o.winner) { HIP_TRY(hipMemcpyAsync(o.winner, J.winner, npix * 4, hipMemcpyDeviceToHost, c->stream)); copied = true; }
-    if (o.rgb) { HIP_TRY(hipMemcpyAsync(o.rgb, J.rgb, npix * 12, hipMemcpyDeviceToHost, c->stream)); copied = true; }
-    if (o.normals) { HIP_TRY(hipMemcpyAsync(o.normals, J.normals, npix * 12, hipMemcpyDeviceToHost, c->stream)); copied = true; }
+    const size_t npix = (size_t)p.jobs[j].W * p.jobs[j].H;
+    void* dst[5] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals};
+    for (int k = 0; k < 5; k++) {
+      if (!dst[k]) continue;
+      HIP_TRY(hipMemcpyAsync(dst[k], (char*)s.sc.outputs.p + offs[j * 5 + k], npix * kOutBytes[k],
+                             hipMemcpyDeviceToHost, s.stream));
+      copied = true;
+    }
   }
-  if (copied) HIP_TRY(hipStreamSynchronize(c->stream));
+  if (copied) HIP_TRY(hipStreamSynchronize(s.stream));
   return MR_OK;
+}
+
+int render_frames(mr_ctx* c, const mr_frame* frames, int n, const mr_outputs* outs) {
+  BatchPlan p;
+  std::vector<size_t> offs;
+  int rc;
+  if ((rc = build_batch(c, frames, n, p))) return rc;
+  if ((rc = bind_outputs(c, c->main, p, outs, offs))) return rc;
+  if ((rc = enqueue_batch(c, c->main, p, c->profiling))) return rc;
+  if ((rc = finish_batch(c, c->main, p, c->profiling))) return rc;
+  return copy_outputs(c, c->main, p, outs, offs);
 }
 
 }  // namespace mr
@@ -440,12 +497,11 @@ int mr_ctx_create(int32_t device, uint64_t hbm_budget, mr_ctx** out) {
   if (!c) return MR_ERR_NOMEM;
   c->device = device;
   c->budget = hbm_budget;
-  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+  if (slot_init(c->main, nullptr) != MR_OK) {
+    slot_free(c, c->main);
     delete c;
     return MR_ERR_HIP;
   }
-  c->stream = c->own_stream;
-  for (auto& e : c->ev) hipEventCreate(&e);
   int cus = 0;
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
   c->num_cus = cus > 0 ? cus : 256;
@@ -456,25 +512,13 @@ int mr_ctx_create(int32_t device, uint64_t hbm_budget, mr_ctx** out) {
 int mr_ctx_destroy(mr_ctx* c) {
   if (!c) return MR_ERR_ARG;
   hipSetDevice(c->device);
-  hipStreamSynchronize(c->stream);
+  hipDeviceSynchronize();
   for (auto& m : c->meshes) {
     dev_free(c, m.pos, false); dev_free(c, m.nrm, false); dev_free(c, m.uv, false);
     dev_free(c, m.idx, false); dev_free(c, m.nidx, false); dev_free(c, m.tidx, false);
   }
   for (auto& t : c->textures) dev_free(c, t.texels, false);
-  Scratch& s = c->sc;
-  DevBuf* bufs[] = {&s.items, &s.region_count, &s.tile_count, &s.tile_off,
-                    &s.tile_cursor, &s.tile_heavy, &s.tile_done, &s.parts, &s.bins,
-                    &s.heavy_keys, &s.scan_status, &s.counters, &s.block_stats, &s.draws,
-                    &s.draw_begin, &s.jobs, &s.job_bases, &s.win, &s.outputs};
-  for (DevBuf* b : bufs) dev_free(c, *b, true);
-  host_free(s.h_draws);
-  host_free(s.h_draw_begin);
-  host_free(s.h_jobs);
-  host_free(s.h_job_bases);
-  host_free(s.h_counters);
-  for (auto& e : c->ev) hipEventDestroy(e);
-  hipStreamDestroy(c->own_stream);
+  slot_free(c, c->main);
   delete c;
   return MR_OK;
 }
@@ -482,19 +526,25 @@ int mr_ctx_destroy(mr_ctx* c) {
 int mr_ctx_set_stream(mr_ctx* c, void* stream) {
   if (!c) return MR_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
-  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  c->main.stream = stream ? (hipStream_t)stream : c->main.own_stream;
   return MR_OK;
 }
 
 int mr_ctx_get_stream(mr_ctx* c, void** stream) {
   if (!c || !stream) return MR_ERR_ARG;
-  *stream = (void*)c->stream;
+  *stream = (void*)c->main.stream;
   return MR_OK;
 }
 
 int mr_ctx_set_tile(mr_ctx* c, int32_t tw, int32_t th) {
   if (!c || !tile_supported(tw, th)) return MR_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (c->tile_w != tw || c->tile_h != th) {
+    // merge buffers are sized per tile: drop them
+    hipStreamSynchronize(c->main.stream);
+    dev_free(c, c->main.sc.heavy_keys, true);
+    c->main.heavy_slots = 0;
+  }
   c->tile_w = tw;
   c->tile_h = th;
   return MR_OK;
@@ -515,14 +565,15 @@ int mr_ctx_set_profiling(mr_ctx* c, int32_t on) {
 int mr_ctx_get_stats(mr_ctx* c, mr_stats* st) {
   if (!c || !st) return MR_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
-  if (c->stats.records < 0 && c->sc.block_stats.p) {
+  Slot& s = c->main;
+  if (c->stats.records < 0 && s.sc.block_stats.p) {
     // Sum the per-workgroup statistics of the last batch.
     std::vector<unsigned long long> h(2 * kStatSlots);
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipMemcpy(h.data(), c->sc.block_stats.p, h.size() * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    HIP_TRY(hipMemcpy(h.data(), s.sc.block_stats.p, h.size() * 8, hipMemcpyDeviceToHost));
     int64_t clipped = 0, records = 0;
-    for (int i = 0; i < c->last_geom_grid; i++) clipped += (int64_t)h[i];
-    for (int i = 0; i < c->last_bin_grid; i++) records += (int64_t)h[kStatSlots + i];
+    for (int i = 0; i < s.last_geom_grid; i++) clipped += (int64_t)h[i];
+    for (int i = 0; i < s.last_bin_grid; i++) records += (int64_t)h[kStatSlots + i];
     c->stats.triangles_clipped = clipped;
     c->stats.records = records;
   }
@@ -597,7 +648,7 @@ int mr_mesh_release(mr_ctx* c, int32_t h) {
   if (!c) return MR_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   if (h < 0 || h >= (int)c->meshes.size() || !c->meshes[h].alive) return MR_ERR_HANDLE;
-  hipStreamSynchronize(c->stream);
+  hipDeviceSynchronize();  // no batch in flight may still read the mesh
   MeshRec& m = c->meshes[h];
   dev_free(c, m.pos, false); dev_free(c, m.nrm, false); dev_free(c, m.uv, false);
   dev_free(c, m.idx, false); dev_free(c, m.nidx, false); dev_free(c, m.tidx, false);
@@ -633,7 +684,7 @@ int mr_texture_release(mr_ctx* c, int32_t h) {
   if (!c) return MR_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   if (h < 0 || h >= (int)c->textures.size() || !c->textures[h].alive) return MR_ERR_HANDLE;
-  hipStreamSynchronize(c->stream);
+  hipDeviceSynchronize();
   dev_free(c, c->textures[h].texels, false);
   c->textures[h].alive = false;
   return MR_OK;

@@ -1,4 +1,4 @@
-// context.h -- host-side state behind the C-ABI (context.cpp).
+// context.h -- host-side state behind the C-ABI (context.cpp, server.cpp).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -38,12 +38,12 @@ struct TexRec {
   bool alive = false;
 };
 
-// Scratch of one batch in flight.
+// Device scratch of one batch in flight, and its pinned staging.
 struct Scratch {
   DevBuf items, region_count, tile_count, tile_off, tile_cursor, tile_heavy,
       tile_done, parts, bins, heavy_keys, scan_status, counters, block_stats,
       draws, draw_begin, jobs, job_bases, win, outputs;
-  HostBuf h_draws, h_draw_begin, h_jobs, h_job_bases, h_counters;
+  HostBuf h_draws, h_draw_begin, h_jobs, h_job_bases, h_counters, h_outputs;
 };
 
 // Jobs and draws of a batch, as the kernels see them.
@@ -58,27 +58,33 @@ struct BatchPlan {
   int64_t shade_blocks = 0;
 };
 
+// An execution slot: scratch, stream and events for one batch in flight.
+struct Slot {
+  Scratch sc;
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  hipEvent_t ev[7] = {};  // stage boundaries (profiling)
+  hipEvent_t done = nullptr;
+  uint32_t heavy_slots = 0;  // merge buffers of split tiles (grown on demand)
+  int32_t last_geom_grid = 0, last_bin_grid = 0;
+  int64_t retries = 0;
+};
+
 }  // namespace mr
 
 struct mr_ctx {
   int32_t device = 0;
-  hipStream_t own_stream = nullptr;
-  hipStream_t stream = nullptr;
   uint64_t budget = 0;
   uint64_t resident_bytes = 0;
   uint64_t scratch_bytes = 0;
   std::vector<mr::MeshRec> meshes;
   std::vector<mr::TexRec> textures;
-  mr::Scratch sc;
+  mr::Slot main;  // synchronous calls (mr_render, mr_render_batch)
   int32_t tile_w = 16, tile_h = 16;
   float bin_factor = 3.0f;
   uint32_t frag_size = 1024;  // max (tile, triangle) pairs per raster item
-  uint32_t heavy_slots = 0;   // merge buffers of split tiles (grown on demand)
   bool profiling = false;
   mr_stats stats{};
-  int32_t last_geom_grid = 0;  // grids of the last batch, to sum block_stats
-  int32_t last_bin_grid = 0;
-  hipEvent_t ev[7] = {};
   int32_t num_cus = 256;
   std::recursive_mutex mu;
 };
@@ -88,9 +94,22 @@ int dev_alloc(mr_ctx* c, DevBuf& b, size_t bytes, bool scratch);
 void dev_free(mr_ctx* c, DevBuf& b, bool scratch);
 int host_alloc(HostBuf& b, size_t bytes);
 void host_free(HostBuf& b);
+int slot_init(Slot& s, hipStream_t external);
+void slot_free(mr_ctx* c, Slot& s);
+// Jobs and draws of n frames (validates handles and sizes).
 int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan);
-BatchArgs make_args(mr_ctx* c, const BatchPlan& p);
-int launch_batch(mr_ctx* c, const BatchArgs& a, bool profile);
-int run_batch(mr_ctx* c, BatchPlan& p);
+// Point the jobs' outputs at device memory: the caller's device pointers, or
+// the slot's staging for host outputs (offsets per job and output kind).
+int bind_outputs(mr_ctx* c, Slot& s, BatchPlan& p, const mr_outputs* outs,
+                 std::vector<size_t>& staging_offs);
+// Upload the batch tables and launch every stage on the slot's stream, then
+// queue the counters' read-back and record s.done.  No host wait.
+int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile);
+// Wait for s.done, handle bin-list / merge-slot overflow by growing and
+// re-running, and record statistics.
+int finish_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile);
+// Copy host outputs back from the slot's staging (synchronous).
+int copy_outputs(mr_ctx* c, Slot& s, const BatchPlan& p, const mr_outputs* outs,
+                 const std::vector<size_t>& staging_offs);
 int render_frames(mr_ctx* c, const mr_frame* frames, int n, const mr_outputs* outs);
 }  // namespace mr

@@ -1,0 +1,261 @@
+// Checks of the minirender C++ API (include/minirender/*.h), written the way
+// a user of the reference library drives it (samples/main.cpp:128-182).
+//
+//   api_check cpu   host-only: primitive counts and order, scene flattening
+//                   order, matrix inverse, projection helpers.
+//   api_check gpu   Renderer::render() on the GPU against the CPU oracle for
+//                   scenes built with the API (nested nodes, transforms,
+//                   materials, a float texture, point light, immediate-mode
+//                   paintMesh after render): winner and depth bit-equal,
+//                   RGB8 within 1 LSB.
+// Prints "ok" and exits 0, or prints the first failure and exits 1.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "minirender/Renderer.h"
+#include "minirender/io.h"
+#include "minirender/primitives.h"
+#include "oracle.h"
+
+using namespace minirender;
+
+#define CHECK(cond, ...)                      \
+  do {                                        \
+    if (!(cond)) {                            \
+      std::printf("FAIL %s:%d: ", __FILE__, __LINE__); \
+      std::printf(__VA_ARGS__);               \
+      std::printf("\n");                      \
+      return 1;                               \
+    }                                         \
+  } while (0)
+
+static int cpu_checks() {
+  // createSphere(r, lat, long): (lat-1)*long + 2 vertices, 2*long*(lat-1) triangles
+  auto s = createSphere(1.0f, 32, 64);
+  CHECK(s->vertices.size() == 31 * 64 + 2, "sphere vertices %zu", s->vertices.size());
+  CHECK(s->indices.size() == 3 * 2 * 64 * 31, "sphere indices %zu", s->indices.size());
+  CHECK(s->indices[0] == 0 && s->indices[1] == 1 && s->indices[2] == 2, "sphere top fan order");
+  CHECK(s->normalsI == s->indices && s->texcoordsI.empty(), "sphere index arrays");
+  auto c = createCube(2.0f);
+  CHECK(c->vertices.size() == 24 && c->indices.size() == 36, "cube sizes");
+  CHECK(c->indices[0] == 0 && c->indices[1] == 2 && c->indices[2] == 1, "cube winding");
+  auto y = createCylinder(0.5f, 1.0f, 16, 2, true);
+  CHECK(y->vertices.size() == 3 * 16 + 2 * 17, "cylinder vertices %zu", y->vertices.size());
+  CHECK(y->indices.size() == 3 * (2 * 16 * 2 + 2 * 16), "cylinder indices");
+  // every cube face is counter-clockwise seen from outside
+  for (size_t t = 0; t < c->indices.size(); t += 3) {
+    const Vec3 a = c->vertices[c->indices[t]], b = c->vertices[c->indices[t + 1]],
+               d = c->vertices[c->indices[t + 2]];
+    const Vec3 n = (b - a) ^ (d - a);
+    CHECK(n * c->normals[c->normalsI[t]] > 0, "cube face %zu winding", t / 3);
+  }
+  // scene flattening: pre-order, a TriMesh before its children (Scene.cpp:13-36)
+  auto scene = std::make_shared<Scene>();
+  auto node = std::make_shared<SceneNode>();
+  auto m1 = createCube(), m2 = createCube(), m3 = createCube();
+  m1->children.push_back(m2);
+  node->children.push_back(m1);
+  scene->add(node);
+  scene->add(m3);
+  node->transform = Matrix4::translate(1, 2, 3);
+  m1->transform = Matrix4::scale(2.0f);
+  Array<Renderable> list;
+  scene->collectShapes(list, Matrix4::identity());
+  CHECK(list.size() == 3 && list[0].mesh == m1.get() && list[1].mesh == m2.get() &&
+            list[2].mesh == m3.get(), "flatten order");
+  CHECK(list[1].transform(0, 3) == 1 && list[1].transform(0, 0) == 2, "flatten transforms");
+  // inverse
+  const Matrix4 M = Matrix4::translate(1, -2, 3) * Matrix4::rotate(Vec3(0.3f, 0.5f, -0.2f)) *
+                    Matrix4::scale(1.7f);
+  const Matrix4 I = M * M.inverse();
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      CHECK(std::fabs(I(i, j) - (i == j ? 1.0f : 0.0f)) < 1e-5f, "inverse %d %d", i, j);
+  // projection helpers agree with the C-ABI
+  const Matrix4 P = projectionFrustum(0.8f, 1.5f, 0.1f, 100.0f);
+  float ref[16];
+  mr_projection_frustum(0.8f, 1.5f, 0.1f, 100.0f, ref);
+  CHECK(std::memcmp(P.data(), ref, sizeof(ref)) == 0, "projectionFrustum");
+  // PPM round trip with the reference quantization
+  Array2<Vec3> img(3, 4);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 4; j++) img(i, j) = Vec3(j / 3.0f, i / 2.0f, 0.5f);
+  const std::string path = "/tmp/mr_api_check.ppm";
+  CHECK(savePPM(img, path), "savePPM");
+  Array2<Vec3> back = loadPPM(path);
+  CHECK(back.rows() == 3 && back.cols() == 4, "loadPPM size");
+  CHECK(back(0, 0).z == 127.0f / 255.0f && back(2, 3).x == 1.0f, "PPM values");
+  std::printf("ok\n");
+  return 0;
+}
+
+// Oracle rendering of draws (from describeFrame) whose mesh handles map to
+// the renderer's TriMeshes; textures given per draw as float arrays.
+struct OracleInput {
+  std::vector<mro_mesh> meshes;
+  std::vector<mro_texture> textures;
+};
+
+static int oracle_render(Renderer& r, const mr_frame& f0, const std::vector<mr_draw>& draws,
+                         const std::vector<const Array2<Vec3>*>& texs, std::vector<float>& depth,
+                         std::vector<uint32_t>& win, std::vector<uint8_t>& rgb8) {
+  OracleInput in;
+  std::vector<mr_draw> d2 = draws;
+  for (size_t i = 0; i < draws.size(); i++) {
+    const TriMesh* m = r.meshOfHandle(draws[i].mesh);
+    if (!m) return MR_ERR_HANDLE;
+    mro_mesh mm{};
+    mm.positions = (const float*)m->vertices.data();
+    mm.npositions = (int32_t)m->vertices.size();
+    mm.normals = (const float*)m->normals.data();
+    mm.nnormals = (int32_t)m->normals.size();
+    mm.indices = m->indices.data();
+    mm.normal_indices = m->normalsI.size() == m->indices.size() ? m->normalsI.data() : m->indices.data();
+    if (!m->texcoords.empty() && m->texcoordsI.size() == m->indices.size()) {
+      mm.uvs = (const float*)m->texcoords.data();
+      mm.nuvs = (int32_t)m->texcoords.size();
+      mm.uv_indices = m->texcoordsI.data();
+    }
+    mm.ntriangles = (int32_t)(m->indices.size() / 3);
+    d2[i].mesh = (int32_t)in.meshes.size();
+    in.meshes.push_back(mm);
+    d2[i].texture = -1;
+    if (texs[i] && texs[i]->rows() > 0) {
+      mro_texture t{texs[i]->data(), MR_TEX_RGB32F, texs[i]->rows(), texs[i]->cols()};
+      d2[i].texture = (int32_t)in.textures.size();
+      in.textures.push_back(t);
+    }
+  }
+  mr_frame f = f0;
+  f.draws = d2.data();
+  f.ndraws = (int32_t)d2.size();
+  const size_t n = (size_t)f.width * f.height;
+  depth.assign(n, 0);
+  win.assign(n, 0);
+  rgb8.assign(n * 3, 0);
+  mr_outputs o{};
+  o.depth = depth.data();
+  o.winner = win.data();
+  o.rgb8 = rgb8.data();
+  return mro_render(&f, in.meshes.data(), (int32_t)in.meshes.size(), in.textures.data(),
+                    (int32_t)in.textures.size(), &o);
+}
+
+static int compare(Renderer& r, const std::vector<float>& depth, const std::vector<uint32_t>& win,
+                   const std::vector<uint8_t>& rgb8, const char* what) {
+  Array2<float> gd = r.getDepth();
+  std::vector<uint32_t> gw = r.getWinners();
+  std::vector<uint8_t> g8 = r.getImageRGB8();
+  CHECK(r.lastStatus() == 0, "%s: render status %d", what, r.lastStatus());
+  size_t covered = 0;
+  for (size_t i = 0; i < win.size(); i++) {
+    CHECK(gw[i] == win[i], "%s: winner at %zu: gpu %u oracle %u", what, i, gw[i], win[i]);
+    CHECK(std::memcmp(&gd.data()[i], &depth[i], 4) == 0, "%s: depth at %zu", what, i);
+    for (int k = 0; k < 3; k++)
+      CHECK(std::abs((int)g8[i * 3 + k] - (int)rgb8[i * 3 + k]) <= 1, "%s: rgb8 at %zu", what, i);
+    covered += win[i] != 0xFFFFFFFFu;
+  }
+  CHECK(covered > 100, "%s: only %zu pixels covered", what, covered);
+  return 0;
+}
+
+static int gpu_checks() {
+  // Scene as in samples/main.cpp: a loaded-style mesh under a node, camera
+  // orbit via setView, frustum projection.
+  auto scene = std::make_shared<Scene>();
+  scene->ambientLight = 0.2f;
+  auto sphere = createSphere(1.0f, 24, 48);
+  sphere->material->diffuse = Vec3(0.7f, 0.5f, 0.3f);
+  sphere->material->shininess = 32;
+  auto cube = createCube(0.8f);
+  cube->transform = Matrix4::translate(1.2f, 0.3f, -0.5f) * Matrix4::rotate(Vec3(0.2f, 0.7f, 0.1f));
+  cube->material->diffuse = Vec3(0.2f, 0.6f, 0.9f);
+  // a textured quad (float texture, uv indices given)
+  auto quad = std::make_shared<TriMesh>();
+  quad->vertices = {Vec3(-2, -1.2f, -1), Vec3(2, -1.2f, -1), Vec3(2, -1.2f, 1.5f), Vec3(-2, -1.2f, 1.5f)};
+  quad->normals = {Vec3(0, 1, 0)};
+  quad->texcoords = {Vec2(0, 0), Vec2(3, 0), Vec2(3, 2), Vec2(0, 2)};
+  quad->indices = {0, 2, 1, 0, 3, 2};
+  quad->normalsI = {0, 0, 0, 0, 0, 0};
+  quad->texcoordsI = {0, 2, 1, 0, 3, 2};
+  quad->material = std::make_shared<Material>();
+  quad->material->texture.resize(16, 16);
+  for (int i = 0; i < 16; i++)
+    for (int j = 0; j < 16; j++)
+      quad->material->texture(i, j) = ((i / 4 + j / 4) & 1) ? Vec3(0.9f, 0.9f, 0.8f) : Vec3(0.2f, 0.3f, 0.5f);
+  auto node = std::make_shared<SceneNode>();
+  node->transform = Matrix4::rotateY(0.3f);
+  node->children.push_back(sphere);
+  sphere->children.push_back(cube);  // cube inherits the sphere's transform
+  scene->add(node);
+  scene->add(quad);
+
+  Renderer r;
+  r.setBackground(Vec3(0.1f, 0.1f, 0.15f));
+  r.setLight(Vec3(2, 3, 4), true);
+  r.setScene(scene);
+  r.setSize(200, 150);
+  r.setProjection(projectionFrustum(0.8f, r.aspect(), 0.1f, 100.0f));
+  for (int frame = 0; frame < 3; frame++) {
+    const float rz = 0.4f * frame;
+    r.setView(Matrix4::translate(0, 0, -5) * Matrix4::rotateX(0.35f) * Matrix4::rotateZ(rz));
+    r.render();
+    const mr_frame f = r.describeFrame();
+    std::vector<mr_draw> draws(f.draws, f.draws + f.ndraws);
+    std::vector<const Array2<Vec3>*> texs;
+    Array<Renderable> list;
+    scene->collectShapes(list, Matrix4::identity());
+    for (auto& it : list) texs.push_back(&it.mesh->material->texture);
+    std::vector<float> depth;
+    std::vector<uint32_t> win;
+    std::vector<uint8_t> rgb8;
+    CHECK(oracle_render(r, f, draws, texs, depth, win, rgb8) == 0, "oracle");
+    if (compare(r, depth, win, rgb8, "frame")) return 1;
+    // immediate mode: paintMesh after render() composites like sequential drawing
+    if (frame == 2) {
+      auto extra = createSphere(0.6f, 12, 24);
+      extra->material->diffuse = Vec3(1, 0.2f, 0.2f);
+      const Matrix4 xf = Matrix4::translate(0.4f, 0.2f, 0.9f);
+      r.paintMesh(extra.get(), xf);
+      r.paintMesh(extra.get(), xf);  // drawn twice: the copy never wins (strict test)
+      // oracle: the frame's draws followed by the two extra draws, one sequence
+      const mr_frame f2 = r.describeFrame();  // re-describes the scene part
+      std::vector<mr_draw> all(f2.draws, f2.draws + f2.ndraws);
+      std::vector<const Array2<Vec3>*> texs2 = texs;
+      // the extra draw as paintMesh builds it (Renderer.cpp:336-338)
+      const Matrix4 view = Matrix4::translate(0, 0, -5) * Matrix4::rotateX(0.35f) * Matrix4::rotateZ(rz);
+      const Matrix4 m4 = view * xf;
+      const Matrix4 nm = m4.inverse().t();
+      mr_draw d{};
+      std::memcpy(d.modelview, m4.data(), 64);
+      std::memcpy(d.normalmat, nm.data(), 64);
+      d.diffuse[0] = 1; d.diffuse[1] = 0.2f; d.diffuse[2] = 0.2f;
+      d.specular[0] = d.specular[1] = d.specular[2] = 0.8f;
+      d.shininess = 12;
+      d.texture = -1;
+      d.mesh = -1;
+      for (int h = 0; h < 4096; h++)
+        if (r.meshOfHandle(h) == extra.get()) { d.mesh = h; break; }
+      CHECK(d.mesh >= 0, "immediate mesh handle");
+      all.push_back(d);
+      all.push_back(d);
+      texs2.push_back(nullptr);
+      texs2.push_back(nullptr);
+      CHECK(oracle_render(r, f2, all, texs2, depth, win, rgb8) == 0, "oracle immediate");
+      if (compare(r, depth, win, rgb8, "immediate")) return 1;
+    }
+  }
+  std::printf("ok\n");
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  const std::string mode = argc > 1 ? argv[1] : "cpu";
+  if (mode == "cpu") return cpu_checks();
+  if (mode == "gpu") return gpu_checks();
+  std::printf("usage: api_check cpu|gpu\n");
+  return 2;
+}
