@@ -1,15 +1,36 @@
 """Benchmark of the rasterization path on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload terrain|megatri|sphere]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload terrain]
+                    [--tile 16] [--no-cpu] [--no-batched] [--no-serving]
 
-A step is one Renderer::render() of the workload (inputs resident in HBM,
-outputs to device buffers).  Prints one JSON line.
+A step is one Renderer::render() of the workload (height-field 1024x1024 +
+10x10 shapes, 2,119,658 triangles, 1920x1080, near clipping active by
+default) with meshes resident in HBM and outputs (RGB8 + fp32 depth) written
+to device buffers.  The K timed steps run through the plain one-request-at-
+a-time path (mr_render) with per-stage HIP events on the library's stream;
+`value` is input triangles per second over the timed region.
+
+Beside it, in the same JSON line:
+  roofline      the dominant kernel's algorithmic HBM bytes per launch over
+                its average event-timed duration, against 8 TB/s;
+  cpu_baseline  the oracle (single-threaded C restatement of the reference
+                renderer) on the same frame on this host, a bounded sample;
+  batched       256 independent jobs (1k-500k triangles, 1-8 materials,
+                320x240..1920x1080, orbit cameras) rendered in one batched
+                pass per step, with the tile size chosen by a short autotune,
+                against the same jobs one mr_render at a time;
+  serving       a Poisson stream over 64 resident meshes (10k-1M triangles)
+                through the continuous-batching service, p50/p99 latency
+                (arrival to outputs in host memory) and frames/s, next to the
+                same stream through a one-request-at-a-time service.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -18,86 +39,373 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "Mtris/s + frames/s at 1080p; p50/p99 ms/request, batched vs plain GPU"
+HBM_PEAK_GBS = 8000.0
+STAGES = ["clear", "geometry", "scan", "bin", "raster", "shade"]
+KERNEL_OF = {"clear": "k_clear", "geometry": "k_geom", "scan": "k_scan", "bin": "k_bin",
+             "raster": "k_tile", "shade": "k_shade"}
+
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--steps", type=int, default=30)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", default="terrain", choices=["terrain", "megatri", "sphere", "textured"])
-    p.add_argument("--tile", type=int, default=16)
+    p.add_argument("--workload", default="terrain",
+                   choices=["terrain", "megatri", "sphere", "textured"])
+    p.add_argument("--tile", type=int, default=0, help="0 = autotune per workload")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--no-serving", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--serve-requests", type=int, default=20000)
+    p.add_argument("--serve-rate", type=float, default=2000.0)
     return p.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def make_frame(name):
     from minirender_amd import scenes
     if name == "terrain":
-        return scenes.config_terrain(), "terrain 1024^2 + 10x10 shapes, 1920x1080"
+        return scenes.config_terrain(), "height-field 1024^2 + 10x10 shapes, 1920x1080, near clip"
     if name == "megatri":
-        return scenes.config_megatri(), "1M-tri grid 708^2, 1920x1080"
+        return scenes.config_megatri(), "1M-triangle grid 708^2, 1920x1080"
     if name == "textured":
-        return scenes.config_textured(), "textured terrain 1024^2 + 2048^2 tex + 5k random tris, 1920x1080"
+        return scenes.config_textured(), \
+            "textured height-field 1024^2 + 2048^2 RGB8 texture + 5k random triangles, 1920x1080"
     return scenes.config_sphere(), "UV sphere 64x32, 640x480"
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
-        sys.exit(2)
-    import torch
-    import minirender_amd as mra
+# ---------------------------------------------------------------------------
+# algorithmic bytes
+# ---------------------------------------------------------------------------
+def input_bytes(frame, with_attrs=True):
+    """Per draw: positions 12 B/vertex + position indices 12 B/triangle; with
+    attributes also normals 12 B, uvs 8 B (when used) and their index arrays
+    when they are distinct arrays."""
+    b = 0
+    for d in frame.draws:
+        m = d.mesh
+        b += 12 * len(m.positions) + 12 * m.ntri
+        if with_attrs:
+            b += 12 * len(m.normals)
+            if m.normal_indices is not None and m.normal_indices is not m.indices:
+                b += 12 * m.ntri
+            if m.has_uv and frame.texturing and d.material.texture is not None:
+                b += 8 * len(m.uvs)
+                if m.uv_indices is not m.indices:
+                    b += 12 * m.ntri
+    return b
 
-    frame, wl = make_frame(args.workload)
-    ctx = mra.Context(0)
-    ctx.set_tile(args.tile, args.tile)
-    dev = torch.device("cuda:0")
+
+def texel_bytes(frame, winner):
+    """3 B per distinct texel sampled by visible pixels is bounded by the
+    texture size; the bound is used (a texture is read at most once)."""
+    b = 0
+    seen = set()
+    for d in frame.draws:
+        t = d.material.texture
+        if t is not None and frame.texturing and id(t) not in seen:
+            seen.add(id(t))
+            b += 3 * t.rows * t.cols
+    return b
+
+
+def kernel_alg_bytes(frame):
+    """Algorithmic HBM bytes per launch of each kernel for one frame."""
+    npx = frame.width * frame.height
+    geo = input_bytes(frame, with_attrs=False)
+    return {
+        "clear": 0,
+        "geometry": geo,       # positions + position indices
+        "scan": 0,
+        "bin": geo,            # the same inputs, re-read to emit (tile, triangle) pairs
+        "raster": 0,           # reads only the bin list (implementation bytes)
+        "shade": npx * (3 + 4) + texel_bytes(frame, None),  # RGB8 + depth out (+ texels)
+    }
+
+
+def frame_alg_bytes(frame):
+    return input_bytes(frame, True) + frame.width * frame.height * 7 + texel_bytes(frame, None)
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary
+    (profiles/pmc_<kernel>_<workload>.txt, line 'traffic_bytes_per_launch: N')."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.txt")
+    try:
+        with open(path) as fh:
+            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", fh.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+# ---------------------------------------------------------------------------
+# sections
+# ---------------------------------------------------------------------------
+def device_outputs(frame, torch, dev):
+    from minirender_amd._capi import mr_outputs
     H, W = frame.height, frame.width
     rgb8 = torch.empty((H, W, 3), dtype=torch.uint8, device=dev)
     depth = torch.empty((H, W), dtype=torch.float32, device=dev)
-    import ctypes as C
-    from minirender_amd._capi import mr_outputs
-    fstruct, keep = ctx.frame(frame)  # host matrices built once, as a caller would
-    outs = mr_outputs()
-    outs.rgb8, outs.depth, outs.on_device = rgb8.data_ptr(), depth.data_ptr(), 1
+    o = mr_outputs()
+    o.rgb8, o.depth, o.on_device = rgb8.data_ptr(), depth.data_ptr(), 1
+    return o, (rgb8, depth)
+
+
+def autotune_tile(ctx, run, tiles=(8, 16, 32), reps=3):
+    """Pick the tile size with the lowest median time of `run()`."""
+    best, table = None, {}
+    for t in tiles:
+        ctx.set_tile(t, t)
+        run()  # warm scratch for this tile size
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            run()
+            ts.append(time.perf_counter() - t0)
+        table[t] = float(np.median(ts))
+        if best is None or table[t] < table[best]:
+            best = t
+    ctx.set_tile(best, best)
+    return best, table
+
+
+def plain_section(args, torch, mra, frame, wl_name):
+    dev = torch.device("cuda:0")
+    ctx = mra.Context(0)
+    fstruct, keep = ctx.frame(frame)
+    outs, bufs = device_outputs(frame, torch, dev)
 
     def step():
         rc = ctx.lib.mr_render(ctx.h, C.byref(fstruct), C.byref(outs))
         if rc:
             raise RuntimeError(f"mr_render failed: {rc}")
 
+    if args.tile:
+        tile, table = args.tile, None
+        ctx.set_tile(tile, tile)
+    else:
+        tile, table = autotune_tile(ctx, step)
+    log(f"plain: tile {tile} (autotune {table})")
     for _ in range(args.warmup):
         step()
+    ctx.set_profiling(True)
+    stage = np.zeros(6)
+    fms = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    ctx.set_profiling(True)
-    stage = np.zeros(6)
-    fms = []
-    for _ in range(10):
-        step()
         s = ctx.stats()
         stage += np.array(s.stage_ms[:6])
         fms.append(s.frame_ms)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
     ctx.set_profiling(False)
     st = ctx.stats()
-    T = frame.ntri
-    out = {
-        "metric": "Mtris/s (plain per-request GPU path)", "value": T / dt / 1e6, "unit": "Mtris/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
-        "frames_per_s": 1.0 / dt, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": wl, "triangles": T, "tile": args.tile},
-        "stage_ms": dict(zip(["clear", "geometry", "scan", "bin", "raster", "shade"], (stage / 10).round(4).tolist())),
-        "frame_ms_gpu": float(np.median(fms)),
-        "records": st.records, "bin_entries": st.bin_entries, "clipped": st.triangles_clipped,
-    }
-    print(json.dumps(out), flush=True)
+    stage /= args.steps
+    alg = kernel_alg_bytes(frame)
+    kernels = {}
+    for i, name in enumerate(STAGES):
+        ms = float(stage[i])
+        gbs = alg[name] / (ms * 1e-3) / 1e9 if ms > 0 and alg[name] else 0.0
+        kernels[KERNEL_OF[name]] = {"ms": round(ms, 5), "alg_bytes": alg[name],
+                                    "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dom = max(range(1, 6), key=lambda i: stage[i])
+    dname = STAGES[dom]
+    dms = float(stage[dom])
+    achieved = alg[dname] / (dms * 1e-3) / 1e9 if dms > 0 else 0.0
+    fb = frame_alg_bytes(frame)
+    fmed = float(np.median(fms))
+    roof = {"bound": "hbm", "kernel": KERNEL_OF[dname], "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "alg_bytes_per_launch": alg[dname], "avg_launch_ms": round(dms, 5),
+            "traffic": read_traffic(KERNEL_OF[dname], wl_name),
+            "frame": {"alg_bytes": fb, "gpu_ms": round(fmed, 5),
+                      "achieved": round(fb / (fmed * 1e-3) / 1e9, 1),
+                      "frac": round(fb / (fmed * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    res = {"ms_per_step": dt * 1e3, "value": frame.ntri / dt / 1e6, "frames_per_s": 1.0 / dt,
+           "tile": tile, "autotune_s": table, "kernels": kernels, "roofline": roof,
+           "records": st.records, "bin_entries": st.bin_entries,
+           "clipped": st.triangles_clipped, "retries": st.retries}
     ctx.close()
+    return res
+
+
+def cpu_section(args, frame, wl):
+    from oracle import oracle_py
+    budget = args.cpu_seconds
+    oracle_py.render(frame, ("rgb8", "depth"))  # warm-up
+    n, t0 = 0, time.perf_counter()
+    while True:
+        oracle_py.render(frame, ("rgb8", "depth"))
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget or (n >= 2 and el * (n + 1) / n > 3 * budget):
+            break
+    per = el / n
+    return {"value": round(frame.ntri / per / 1e6, 3), "unit": "Mtris/s", "cores": 1,
+            "kind": "port", "frames_per_s": round(1.0 / per, 4),
+            "sample": f"{n} frames of the bench workload ({wl}) after 1 warm-up frame, "
+                      f"{el:.1f} s, one thread, oracle/liboracle.so (-O3 -ffp-contract=off)"}
+
+
+def batched_section(args, torch, mra):
+    from minirender_amd import scenes
+    from minirender_amd._capi import mr_frame, mr_outputs
+    t_gen = time.perf_counter()
+    jobs = scenes.batch_jobs()
+    log(f"batched: generated {len(jobs)} jobs in {time.perf_counter() - t_gen:.1f} s")
+    dev = torch.device("cuda:0")
+    ctx = mra.Context(0)
+    structs, keep, outs, bufs = [], [], [], []
+    for fr in jobs:
+        s, k = ctx.frame(fr)
+        o, b = device_outputs(fr, torch, dev)
+        structs.append(s)
+        keep.append(k)
+        outs.append(o)
+        bufs.append(b)
+    n = len(jobs)
+    farr = (mr_frame * n)(*structs)
+    oarr = (mr_outputs * n)(*outs)
+    tris = sum(fr.ntri for fr in jobs)
+
+    def batch():
+        rc = ctx.lib.mr_render_batch(ctx.h, farr, n, oarr)
+        if rc:
+            raise RuntimeError(f"mr_render_batch failed: {rc}")
+
+    def plain():
+        for i in range(n):
+            rc = ctx.lib.mr_render(ctx.h, C.byref(farr[i]), C.byref(oarr[i]))
+            if rc:
+                raise RuntimeError(f"mr_render failed: {rc}")
+
+    tile, table = autotune_tile(ctx, batch)
+    reps = max(3, args.steps // 5)
+    batch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch()
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / reps
+    plain()
+    t0 = time.perf_counter()
+    for _ in range(max(1, reps // 2)):
+        plain()
+    torch.cuda.synchronize()
+    tp = (time.perf_counter() - t0) / max(1, reps // 2)
+    ctx.close()
+    return {"jobs": n, "triangles": tris, "tile": tile, "autotune_s": table,
+            "batched": {"ms_per_pass": tb * 1e3, "frames_per_s": n / tb, "Mtris_per_s": tris / tb / 1e6},
+            "plain": {"ms_per_pass": tp * 1e3, "frames_per_s": n / tp, "Mtris_per_s": tris / tp / 1e6},
+            "speedup": tp / tb}
+
+
+def serving_section(args, torch, mra):
+    from minirender_amd import scenes
+    from minirender_amd._capi import mr_frame, mr_outputs
+    t_gen = time.perf_counter()
+    pool = scenes.serving_pool()
+    reqs = scenes.serving_requests(pool, args.serve_requests)
+    log(f"serving: pool of {len(pool)} meshes, {len(reqs)} requests, "
+        f"generated in {time.perf_counter() - t_gen:.1f} s")
+    ctx = mra.Context(0)
+    ctx.set_tile(16, 16)
+    # host outputs: one buffer set per resolution, shared by all requests of
+    # that resolution (the service writes them in completion order)
+    shared = {}
+    structs, keep, outs = [], [], []
+    for fr in reqs:
+        s, k = ctx.frame(fr)
+        key = (fr.width, fr.height)
+        if key not in shared:
+            shared[key] = mra.alloc_outputs(fr, ("rgb8", "depth"))
+        structs.append(s)
+        keep.append(k)
+        outs.append(mra.outputs_struct(shared[key]))
+    n = len(reqs)
+    farr = (mr_frame * n)(*structs)
+    oarr = (mr_outputs * n)(*outs)
+    rng = np.random.default_rng(12)
+    gaps = rng.exponential(1.0 / args.serve_rate, n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+
+    def trace(max_batch, slots, arr, count):
+        with mra.Server(ctx, max_batch=max_batch, slots=slots,
+                        hbm_budget=32 * (1 << 30)) as srv:
+            lat, mk = srv.run_trace(farr, list(arr[:count]), oarr)
+        return {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "frames_per_s": round(count / mk, 1), "requests": count,
+                "offered_rate": round(count / max(arr[count - 1], 1e-9), 1)}
+
+    # warm both configurations (scratch sized for the largest batches)
+    trace(64, 3, np.zeros(n), min(n, 512))
+    modes = {"plain": (1, 1), "batched": (64, 3)}
+    out = {"rate": args.serve_rate, "pool_meshes": len(pool), "hbm_budget_gb": 32,
+           "outputs": "host (RGB8 + fp32 depth)"}
+    for name, (mb, sl) in modes.items():
+        out[name] = trace(mb, sl, arrivals, n)
+        log(f"serving {name}: {out[name]}")
+    # capacity: every request queued at t=0
+    for name, (mb, sl) in modes.items():
+        m = min(n, 4000)
+        r = trace(mb, sl, np.zeros(n), m)
+        out[name]["saturated_frames_per_s"] = r["frames_per_s"]
+        out[name]["saturated_p99_ms"] = r["p99_ms"]
+    out["saturated_speedup"] = round(out["batched"]["saturated_frames_per_s"] /
+                                     out["plain"]["saturated_frames_per_s"], 3)
+    ctx.close()
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this implementation runs on one GPU; the sharded design "
+                                   "is described but not built, so --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    import minirender_amd as mra
+
+    frame, wl = make_frame(args.workload)
+    log(f"workload {args.workload}: {frame.ntri} triangles")
+    plain = plain_section(args, torch, mra, frame, args.workload)
+    log(f"plain: {plain['ms_per_step']:.3f} ms/step")
+    res = {
+        "metric": METRIC, "value": round(plain["value"], 2), "unit": "Mtris/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(plain["ms_per_step"], 4),
+        "frames_per_s": round(plain["frames_per_s"], 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded generators)",
+        "config": {"workload": wl, "triangles": frame.ntri, "width": frame.width,
+                   "height": frame.height, "tile": plain["tile"], "path": "plain mr_render",
+                   "outputs": "device (RGB8 + fp32 depth)"},
+        "roofline": plain["roofline"],
+        "kernels": plain["kernels"],
+        "stats": {"records": plain["records"], "bin_entries": plain["bin_entries"],
+                  "clipped": plain["clipped"], "retries": plain["retries"]},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_section(args, frame, wl)
+        log(f"cpu: {res['cpu_baseline']}")
+    if not args.no_batched:
+        res["batched"] = batched_section(args, torch, mra)
+        log(f"batched: {res['batched']}")
+    if not args.no_serving:
+        res["serving"] = serving_section(args, torch, mra)
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

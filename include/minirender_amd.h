@@ -118,6 +118,12 @@ int mr_ctx_set_tile(mr_ctx* ctx, int32_t tile_w, int32_t tile_h);
 /* Bin-list capacity in (tile, triangle) entries per input triangle
  * (>= 1); overflow re-runs the frame with a larger list. */
 int mr_ctx_set_bin_factor(mr_ctx* ctx, float entries_per_triangle);
+/* Capture the steady state as HIP graphs: a batch whose launch parameters
+ * (sizes, scratch, tile configuration) equal the previous one on the same
+ * slot replays the captured uploads + kernels instead of launching them one
+ * by one.  Off by default; ignored while profiling or on an external stream.
+ * No counterpart in the reference (host-side launch overhead). */
+int mr_ctx_set_graphs(mr_ctx* ctx, int32_t on);
 /* Record per-stage HIP event times into mr_stats (adds event overhead). */
 int mr_ctx_set_profiling(mr_ctx* ctx, int32_t on);
 int mr_ctx_get_stats(mr_ctx* ctx, mr_stats* stats);

@@ -117,6 +117,9 @@ class Context:
     def set_profiling(self, on: bool) -> None:
         check(self.lib.mr_ctx_set_profiling(self.h, int(on)), "mr_ctx_set_profiling")
 
+    def set_graphs(self, on: bool) -> None:
+        check(self.lib.mr_ctx_set_graphs(self.h, int(on)), "mr_ctx_set_graphs")
+
     def set_stream(self, stream: int) -> None:
         check(self.lib.mr_ctx_set_stream(self.h, C.c_void_p(stream)), "mr_ctx_set_stream")
 

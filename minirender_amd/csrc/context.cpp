@@ -114,6 +114,7 @@ void slot_free(mr_ctx* c, Slot& s) {
   HostBuf* hbufs[] = {&x.h_draws, &x.h_draw_begin, &x.h_jobs, &x.h_job_bases,
                       &x.h_counters, &x.h_outputs};
   for (HostBuf* b : hbufs) host_free(*b);
+  if (s.graph) hipGraphExecDestroy(s.graph);
   for (auto& e : s.ev)
     if (e) hipEventDestroy(e);
   if (s.done) hipEventDestroy(s.done);
@@ -330,27 +331,72 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
     J.win = win;
     win += (size_t)J.W * J.H;
   }
+  // Host tables go to pinned staging first; the uploads read them from there
+  // (also on a graph replay, which re-reads the same staging).
   memcpy(x.h_jobs.p, p.jobs.data(), p.jobs.size() * sizeof(JobDesc));
-  HIP_TRY(hipMemcpyAsync(x.jobs.p, x.h_jobs.p, p.jobs.size() * sizeof(JobDesc),
-                         hipMemcpyHostToDevice, s.stream));
   memcpy(x.h_job_bases.p, p.job_bases.data(), p.job_bases.size() * 4);
-  HIP_TRY(hipMemcpyAsync(x.job_bases.p, x.h_job_bases.p, p.job_bases.size() * 4,
-                         hipMemcpyHostToDevice, s.stream));
   if (!p.draws.empty()) {
     memcpy(x.h_draws.p, p.draws.data(), p.draws.size() * sizeof(DrawDesc));
     memcpy(x.h_draw_begin.p, p.draw_begin.data(), p.draws.size() * 8);
-    HIP_TRY(hipMemcpyAsync(x.draws.p, x.h_draws.p, p.draws.size() * sizeof(DrawDesc),
-                           hipMemcpyHostToDevice, s.stream));
-    HIP_TRY(hipMemcpyAsync(x.draw_begin.p, x.h_draw_begin.p, p.draws.size() * 8,
-                           hipMemcpyHostToDevice, s.stream));
   }
   const BatchArgs a = make_args(c, s, p);
-  if ((rc = launch_stages(a, s, profile))) return rc;
-  HIP_TRY(hipMemcpyAsync(x.h_counters.p, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost,
-                         s.stream));
-  HIP_TRY(hipEventRecord(s.done, s.stream));
   s.last_geom_grid = p.total_tris ? geom_grid(a) : 0;
   s.last_bin_grid = bin_grid(a);
+  auto issue = [&]() -> int {
+    HIP_TRY(hipMemcpyAsync(x.jobs.p, x.h_jobs.p, p.jobs.size() * sizeof(JobDesc),
+                           hipMemcpyHostToDevice, s.stream));
+    HIP_TRY(hipMemcpyAsync(x.job_bases.p, x.h_job_bases.p, p.job_bases.size() * 4,
+                           hipMemcpyHostToDevice, s.stream));
+    if (!p.draws.empty()) {
+      HIP_TRY(hipMemcpyAsync(x.draws.p, x.h_draws.p, p.draws.size() * sizeof(DrawDesc),
+                             hipMemcpyHostToDevice, s.stream));
+      HIP_TRY(hipMemcpyAsync(x.draw_begin.p, x.h_draw_begin.p, p.draws.size() * 8,
+                             hipMemcpyHostToDevice, s.stream));
+    }
+    int r = launch_stages(a, s, profile);
+    if (r) return r;
+    HIP_TRY(hipMemcpyAsync(x.h_counters.p, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost,
+                           s.stream));
+    return MR_OK;
+  };
+  // Graphs only on the slot's own stream (a caller's stream may be the null
+  // stream, which cannot be captured) and without per-stage events.
+  if (c->graphs && !profile && s.own_stream && s.stream == s.own_stream) {
+    const void* hp[5] = {x.h_jobs.p, x.h_job_bases.p, x.h_draws.p, x.h_draw_begin.p,
+                         x.h_counters.p};
+    std::vector<unsigned char> key(sizeof(BatchArgs) + sizeof(hp));
+    memcpy(key.data(), &a, sizeof(BatchArgs));
+    memcpy(key.data() + sizeof(BatchArgs), hp, sizeof(hp));
+    if (!s.graph || key != s.graph_key) {
+      if (s.graph) {
+        hipGraphExecDestroy(s.graph);
+        s.graph = nullptr;
+      }
+      s.graph_key.clear();
+      HIP_TRY(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+      const int r = issue();
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(s.stream, &g);
+      if (r) {
+        if (g) hipGraphDestroy(g);
+        return r;
+      }
+      if (e != hipSuccess) return MR_ERR_HIP;
+      const hipError_t ei = hipGraphInstantiate(&s.graph, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (ei != hipSuccess) {
+        s.graph = nullptr;
+        return MR_ERR_HIP;
+      }
+      s.graph_key = std::move(key);
+    } else {
+      s.graph_replays++;
+    }
+    HIP_TRY(hipGraphLaunch(s.graph, s.stream));
+  } else {
+    if ((rc = issue())) return rc;
+  }
+  HIP_TRY(hipEventRecord(s.done, s.stream));
   return MR_OK;
 }
 
@@ -559,6 +605,13 @@ int mr_ctx_set_bin_factor(mr_ctx* c, float f) {
 int mr_ctx_set_profiling(mr_ctx* c, int32_t on) {
   if (!c) return MR_ERR_ARG;
   c->profiling = on != 0;
+  return MR_OK;
+}
+
+int mr_ctx_set_graphs(mr_ctx* c, int32_t on) {
+  if (!c) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  c->graphs = on != 0;
   return MR_OK;
 }
 

@@ -68,6 +68,11 @@ struct Slot {
   uint32_t heavy_slots = 0;  // merge buffers of split tiles (grown on demand)
   int32_t last_geom_grid = 0, last_bin_grid = 0;
   int64_t retries = 0;
+  // Steady state: the last batch's uploads + stages + counter read-back as
+  // an instantiated graph, replayed while the launch key is unchanged.
+  hipGraphExec_t graph = nullptr;
+  std::vector<unsigned char> graph_key;
+  int64_t graph_replays = 0;
 };
 
 }  // namespace mr
@@ -84,6 +89,7 @@ struct mr_ctx {
   float bin_factor = 3.0f;
   uint32_t frag_size = 1024;  // max (tile, triangle) pairs per raster item
   bool profiling = false;
+  bool graphs = false;  // capture steady-state batches as HIP graphs
   mr_stats stats{};
   int32_t num_cus = 256;
   std::recursive_mutex mu;

@@ -301,50 +301,52 @@ def create_cylinder(radius: float, height: float, segments: int = 32,
 
 def create_sphere(radius: float, lat_segments: int = 16, long_segments: int = 32) -> Mesh:
     """createSphere, primitives.cpp:150-230.  The top fan (0, i1, i2) is wound
-    opposite to the bands, so it is back-face culled seen from outside."""
+    opposite to the bands, so it is back-face culled seen from outside.
+    sin/cos are evaluated per ring and per meridian in double and rounded to
+    fp32; every product is an fp32 operation in the order x = (r sin phi) cos th."""
     lat_segments = max(lat_segments, 2)
     long_segments = max(long_segments, 3)
     radius = F32(radius)
-    pos = [(0.0, radius, 0.0)]
-    nrm = [(0.0, 1.0, 0.0)]
-    uv = [(0.5, 0.0)]
-    first = 1
-    for lat in range(1, lat_segments):
-        v = F32(lat) / F32(lat_segments)
-        phi = v * PIF
-        sp, cp = F32(math.sin(phi)), F32(math.cos(phi))
-        for lon in range(long_segments):
-            u = F32(lon) / F32(long_segments)
-            th = u * F32(2.0) * PIF
-            x = radius * sp * F32(math.cos(th))
-            y = radius * cp
-            z = radius * sp * F32(math.sin(th))
-            pos.append((x, y, z))
-            nrm.append(normalized((x, y, z)))
-            uv.append((u, F32(1.0) - v))
-    bottom = len(pos)
-    pos.append((0.0, -radius, 0.0))
-    nrm.append((0.0, -1.0, 0.0))
-    uv.append((0.5, 1.0))
-    idx = []
-    for lon in range(long_segments):
-        idx.append((0, first + lon, first + (lon + 1) % long_segments))
+    L = long_segments
+    lats = np.arange(1, lat_segments)
+    v = lats.astype(F32) / F32(lat_segments)
+    phi = v * PIF
+    sp = np.array([math.sin(float(p)) for p in phi], F32)
+    cp = np.array([math.cos(float(p)) for p in phi], F32)
+    u = np.arange(L).astype(F32) / F32(L)
+    th = u * F32(2.0) * PIF
+    ct = np.array([math.cos(float(t)) for t in th], F32)
+    st = np.array([math.sin(float(t)) for t in th], F32)
+    rs = (radius * sp)[:, None]
+    x = rs * ct[None, :]
+    y = np.broadcast_to((radius * cp)[:, None], x.shape)
+    z = rs * st[None, :]
+    ring = np.stack([x, y, z], -1).reshape(-1, 3).astype(F32)
+    ln = np.sqrt((ring[:, 0] * ring[:, 0] + ring[:, 1] * ring[:, 1]) + ring[:, 2] * ring[:, 2])
+    ring_n = (ring / ln[:, None]).astype(F32)
+    ring_uv = np.stack([np.broadcast_to(u[None, :], x.shape),
+                        np.broadcast_to((F32(1.0) - v)[:, None], x.shape)], -1).reshape(-1, 2)
+    pos = np.concatenate([np.array([[0.0, radius, 0.0]], F32), ring,
+                          np.array([[0.0, -radius, 0.0]], F32)])
+    nrm = np.concatenate([np.array([[0.0, 1.0, 0.0]], F32), ring_n,
+                          np.array([[0.0, -1.0, 0.0]], F32)])
+    uv = np.concatenate([np.array([[0.5, 0.0]], F32), ring_uv.astype(F32),
+                         np.array([[0.5, 1.0]], F32)])
+    first, bottom = 1, len(pos) - 1
+    lon = np.arange(L)
+    nxt = (lon + 1) % L
+    parts = [np.stack([np.zeros(L, np.int64), first + lon, first + nxt], -1)]
     rings = lat_segments - 1
     if rings >= 2:
-        for lat in range(rings - 1):
-            row1 = first + lat * long_segments
-            row2 = row1 + long_segments
-            for lon in range(long_segments):
-                a = row1 + lon
-                b = row1 + (lon + 1) % long_segments
-                c = row2 + lon
-                d = row2 + (lon + 1) % long_segments
-                idx += [(a, b, c), (b, d, c)]
-    last = first + (rings - 1) * long_segments
-    for lon in range(long_segments):
-        idx.append((bottom, last + lon, last + (lon + 1) % long_segments))
-    return Mesh(np.array(pos, F32), np.array(nrm, F32), np.array(idx, np.int32),
-                uvs=np.array(uv, F32))
+        row1 = first + np.arange(rings - 1)[:, None] * L
+        row2 = row1 + L
+        a, b = row1 + lon[None, :], row1 + nxt[None, :]
+        c, d = row2 + lon[None, :], row2 + nxt[None, :]
+        band = np.stack([np.stack([a, b, c], -1), np.stack([b, d, c], -1)], 2)
+        parts.append(band.reshape(-1, 3))
+    last = first + (rings - 1) * L
+    parts.append(np.stack([np.full(L, bottom), last + lon, last + nxt], -1))
+    return Mesh(pos, nrm, np.concatenate(parts).astype(np.int32), uvs=uv)
 
 
 # ---------------------------------------------------------------------------
@@ -542,6 +544,38 @@ def split_materials(mesh: Mesh, parts: int, rng: np.random.Generator) -> List[Dr
                        shininess=float(rng.choice([0.0, 8.0, 16.0, 32.0])))
         draws.append(Draw(m, mat))
     return draws
+
+
+def serving_pool(nmeshes: int = 64, seed: int = 9, tri_lo: int = 10_000,
+                 tri_hi: int = 1_000_000) -> List[List[Draw]]:
+    """Config 5's resident pool: `nmeshes` seeded meshes (height-fields or
+    spheres) of log-uniform triangle count, each split into 1-8 materials.
+    Returns one draw list per mesh."""
+    rng = np.random.default_rng(seed)
+    pool = []
+    for _ in range(nmeshes):
+        t = int(math.exp(rng.uniform(math.log(tri_lo), math.log(tri_hi))))
+        if rng.random() < 0.5:
+            mesh = height_field(grid_side_for(t), int(rng.integers(1 << 30)), 0.3)
+        else:
+            lat = max(3, int(math.sqrt(t / 4)))
+            mesh = create_sphere(1.0, lat, 2 * lat)
+        pool.append(split_materials(mesh, int(rng.integers(1, 9)), rng))
+    return pool
+
+
+def serving_requests(pool: List[List[Draw]], n: int, seed: int = 10) -> List[FrameSpec]:
+    """`n` requests over a resident pool: a uniformly chosen mesh with its own
+    camera on a sphere of radius 2-5, point light and resolution."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        draws = pool[int(rng.integers(0, len(pool)))]
+        w, h = RESOLUTIONS[int(rng.integers(0, len(RESOLUTIONS)))]
+        light = tuple(float(v) for v in rng.uniform(-4.0, 4.0, 3))
+        out.append(FrameSpec(w, h, draws=draws, light=light, light_is_point=True, ambient=0.1,
+                             **orbit_camera(rng, w, h)))
+    return out
 
 
 def batch_jobs(njobs: int = 256, seed: int = 5, tri_lo: int = 1000, tri_hi: int = 500_000) -> List[FrameSpec]:
