@@ -115,6 +115,10 @@ int mr_ctx_get_stream(mr_ctx* ctx, void** stream);
 /* Tile configuration: tile_w x tile_h pixels per raster workgroup
  * (each of 8, 16, 32).  Defaults to 16 x 16. */
 int mr_ctx_set_tile(mr_ctx* ctx, int32_t tile_w, int32_t tile_h);
+/* Workgroup size of the tile raster for the current tile size (0 = the
+ * default; 8x8: 64 or 128, 16x16: 128, 256 or 512, 32x32: 256 or 512).
+ * mr_ctx_set_tile resets it to the default. */
+int mr_ctx_set_raster_threads(mr_ctx* ctx, int32_t threads);
 /* Bin-list capacity in (tile, triangle) entries per input triangle
  * (>= 1); overflow re-runs the frame with a larger list. */
 int mr_ctx_set_bin_factor(mr_ctx* ctx, float entries_per_triangle);

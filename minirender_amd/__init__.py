@@ -111,6 +111,9 @@ class Context:
     def set_tile(self, tw: int, th: int) -> None:
         check(self.lib.mr_ctx_set_tile(self.h, tw, th), "mr_ctx_set_tile")
 
+    def set_raster_threads(self, n: int) -> None:
+        check(self.lib.mr_ctx_set_raster_threads(self.h, n), "mr_ctx_set_raster_threads")
+
     def set_bin_factor(self, f: float) -> None:
         check(self.lib.mr_ctx_set_bin_factor(self.h, f), "mr_ctx_set_bin_factor")
 

@@ -289,6 +289,7 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.ntiles = (int32_t)p.ntiles;
   a.tile_w = c->tile_w;
   a.tile_h = c->tile_h;
+  a.raster_threads = c->raster_threads;
   a.scan_blocks = scan_blocks_for((int32_t)p.ntiles);
   a.shade_blocks = (int32_t)p.shade_blocks;
   a.num_cus = c->num_cus;
@@ -593,6 +594,16 @@ int mr_ctx_set_tile(mr_ctx* c, int32_t tw, int32_t th) {
   }
   c->tile_w = tw;
   c->tile_h = th;
+  c->raster_threads = default_raster_threads(tw);
+  return MR_OK;
+}
+
+int mr_ctx_set_raster_threads(mr_ctx* c, int32_t threads) {
+  if (!c) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (threads == 0) threads = default_raster_threads(c->tile_w);
+  if (!raster_threads_supported(c->tile_w, threads)) return MR_ERR_ARG;
+  c->raster_threads = threads;
   return MR_OK;
 }
 

@@ -86,6 +86,7 @@ struct mr_ctx {
   std::vector<mr::TexRec> textures;
   mr::Slot main;  // synchronous calls (mr_render, mr_render_batch)
   int32_t tile_w = 16, tile_h = 16;
+  int32_t raster_threads = 256;  // tile raster workgroup size
   float bin_factor = 3.0f;
   uint32_t frag_size = 1024;  // max (tile, triangle) pairs per raster item
   bool profiling = false;

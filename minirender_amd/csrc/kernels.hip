@@ -715,18 +715,36 @@ bool tile_supported(int tw, int th) {
   return (tw == 8 && th == 8) || (tw == 16 && th == 16) || (tw == 32 && th == 32);
 }
 
+int default_raster_threads(int tw) { return tw == 8 ? 64 : 256; }
+
+bool raster_threads_supported(int tw, int nt) {
+  switch (tw) {
+    case 8: return nt == 64 || nt == 128;
+    case 16: return nt == 128 || nt == 256 || nt == 512;
+    case 32: return nt == 256 || nt == 512;
+    default: return false;
+  }
+}
+
+#define MR_TILE_CASE(TW, NT)                                                         \
+  if (a.tile_w == TW && a.raster_threads == NT) {                                     \
+    hipLaunchKernelGGL((k_tile<TW, TW, NT>), dim3(grid), dim3(NT), 0, s, a);          \
+    return hipGetLastError();                                                         \
+  }
+
 hipError_t launch_tile(const BatchArgs& a, hipStream_t s) {
   const int grid = a.ntiles + (int)a.parts_cap;
-  if (a.tile_w == 8 && a.tile_h == 8)
-    hipLaunchKernelGGL((k_tile<8, 8, 64>), dim3(grid), dim3(64), 0, s, a);
-  else if (a.tile_w == 16 && a.tile_h == 16)
-    hipLaunchKernelGGL((k_tile<16, 16, 256>), dim3(grid), dim3(256), 0, s, a);
-  else if (a.tile_w == 32 && a.tile_h == 32)
-    hipLaunchKernelGGL((k_tile<32, 32, 256>), dim3(grid), dim3(256), 0, s, a);
-  else
-    return hipErrorInvalidValue;
-  return hipGetLastError();
+  if (a.tile_w != a.tile_h) return hipErrorInvalidValue;
+  MR_TILE_CASE(8, 64)
+  MR_TILE_CASE(8, 128)
+  MR_TILE_CASE(16, 128)
+  MR_TILE_CASE(16, 256)
+  MR_TILE_CASE(16, 512)
+  MR_TILE_CASE(32, 256)
+  MR_TILE_CASE(32, 512)
+  return hipErrorInvalidValue;
 }
+#undef MR_TILE_CASE
 
 hipError_t launch_shade(const BatchArgs& a, hipStream_t s) {
   const int grid = a.shade_blocks;

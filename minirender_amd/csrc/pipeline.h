@@ -40,6 +40,7 @@ struct BatchArgs {
   int64_t total_tris;
   int32_t ntiles;
   int32_t tile_w, tile_h;    // each 8, 16 or 32
+  int32_t raster_threads;    // workgroup size of the tile raster
   int32_t scan_blocks;
   int32_t shade_blocks;
   int32_t num_cus;
@@ -71,5 +72,9 @@ hipError_t launch_bin(const BatchArgs& a, hipStream_t s);
 hipError_t launch_tile(const BatchArgs& a, hipStream_t s);
 hipError_t launch_shade(const BatchArgs& a, hipStream_t s);
 bool tile_supported(int tile_w, int tile_h);
+// Raster workgroup sizes built per tile size: 8x8 {64, 128}, 16x16
+// {128, 256, 512}, 32x32 {256, 512}; the default is 64 for 8x8, else 256.
+bool raster_threads_supported(int tile_w, int threads);
+int default_raster_threads(int tile_w);
 
 }  // namespace mr

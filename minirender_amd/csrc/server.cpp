@@ -51,6 +51,7 @@ struct Flight {
   std::vector<Request*> reqs;
   std::vector<mr_outputs> outs;  // device staging per request
   std::vector<size_t> offs;
+  std::vector<char> staged;  // per request x output kind: read back via staging
   uint64_t mem = 0;
   int64_t batch_id = -1;
 };
@@ -98,8 +99,60 @@ struct mr_server {
   }
   void run();
   bool launch(Flight& f, std::vector<Request*>& reqs);
+  int queue_readback(Flight& f);
   void complete(Flight& f);
 };
+
+// True for host memory the HIP runtime has pinned (hipHostMalloc, torch's
+// pin_memory): the device can copy straight into it.
+static bool is_pinned(const void* p) {
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeHost;
+}
+
+static const size_t kOutBytes[5] = {3, 4, 4, 12, 12};  // rgb8 depth winner rgb normals
+
+// Queue the read-back of every host output of the batch on the slot's
+// stream: pinned destinations directly, the rest into pinned staging (copied
+// out by complete()).  Marks which outputs need the staging copy.
+int mr_server::queue_readback(Flight& f) {
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  f.staged.assign(f.reqs.size() * 5, 0);
+  size_t need = 0;
+  for (size_t j = 0; j < f.reqs.size(); j++) {
+    const mr_outputs& o = f.reqs[j]->out;
+    if (o.on_device) continue;
+    const size_t npix = (size_t)f.plan.jobs[j].W * f.plan.jobs[j].H;
+    const void* dst[5] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals};
+    for (int k = 0; k < 5; k++)
+      if (dst[k] && !is_pinned(dst[k])) {
+        f.staged[j * 5 + k] = 1;
+        need = std::max(need, f.offs[j * 5 + k] + npix * kOutBytes[k]);
+      }
+  }
+  if (need && host_alloc(f.slot.sc.h_outputs, need)) return MR_ERR_NOMEM;
+  for (size_t j = 0; j < f.reqs.size(); j++) {
+    const mr_outputs& o = f.reqs[j]->out;
+    if (o.on_device) continue;
+    const size_t npix = (size_t)f.plan.jobs[j].W * f.plan.jobs[j].H;
+    void* dst[5] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals};
+    for (int k = 0; k < 5; k++) {
+      if (!dst[k]) continue;
+      const size_t off = f.offs[j * 5 + k];
+      void* to = f.staged[j * 5 + k] ? (char*)f.slot.sc.h_outputs.p + off : dst[k];
+      if (hipMemcpyAsync(to, (const char*)f.slot.sc.outputs.p + off, npix * kOutBytes[k],
+                         hipMemcpyDeviceToHost, f.slot.stream) != hipSuccess) {
+        (void)hipGetLastError();
+        return MR_ERR_HIP;
+      }
+    }
+  }
+  return MR_OK;
+}
 
 bool mr_server::launch(Flight& f, std::vector<Request*>& reqs) {
   std::vector<mr_frame> frames;
@@ -109,6 +162,7 @@ bool mr_server::launch(Flight& f, std::vector<Request*>& reqs) {
     frames.push_back(reqs[i]->frame);
     f.outs[i] = reqs[i]->out;
   }
+  f.reqs = reqs;
   int rc;
   {
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -116,16 +170,8 @@ bool mr_server::launch(Flight& f, std::vector<Request*>& reqs) {
     rc = build_batch(ctx, frames.data(), (int)frames.size(), f.plan);
     if (!rc) rc = bind_outputs(ctx, f.slot, f.plan, f.outs.data(), f.offs);
     if (!rc) rc = enqueue_batch(ctx, f.slot, f.plan, false);
-    if (!rc) {
-      // read-back of host outputs into pinned staging, in the same stream
-      const size_t need = f.slot.sc.outputs.bytes;
-      rc = host_alloc(f.slot.sc.h_outputs, need);
-      if (!rc && need)
-        if (hipMemcpyAsync(f.slot.sc.h_outputs.p, f.slot.sc.outputs.p, need,
-                           hipMemcpyDeviceToHost, f.slot.stream) != hipSuccess)
-          rc = MR_ERR_HIP;
-      if (!rc && hipEventRecord(f.slot.done, f.slot.stream) != hipSuccess) rc = MR_ERR_HIP;
-    }
+    if (!rc) rc = queue_readback(f);
+    if (!rc && hipEventRecord(f.slot.done, f.slot.stream) != hipSuccess) rc = MR_ERR_HIP;
   }
   const double t = now_us();
   f.batch_id = next_batch++;
@@ -134,8 +180,8 @@ bool mr_server::launch(Flight& f, std::vector<Request*>& reqs) {
     r->batch_id = f.batch_id;
     r->batch_size = (int32_t)reqs.size();
   }
-  f.reqs = reqs;
   if (rc) {  // fail the whole batch
+    if (f.slot.stream) hipStreamSynchronize(f.slot.stream);
     std::lock_guard<std::mutex> lk(mu);
     for (Request* r : reqs) {
       r->status = rc;
@@ -143,6 +189,7 @@ bool mr_server::launch(Flight& f, std::vector<Request*>& reqs) {
       r->done_us = now_us();
     }
     mem_in_flight -= f.mem;
+    f.mem = 0;
     f.reqs.clear();
     cv_done.notify_all();
     return false;
@@ -157,22 +204,21 @@ void mr_server::complete(Flight& f) {
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     hipSetDevice(ctx->device);
     rc = finish_batch(ctx, f.slot, f.plan, false);
-    if (!rc && f.slot.retries && f.slot.sc.outputs.bytes) {
+    if (!rc && f.slot.retries) {
       // re-run after an overflow: read the outputs back again
-      if (hipMemcpy(f.slot.sc.h_outputs.p, f.slot.sc.outputs.p, f.slot.sc.outputs.bytes,
-                    hipMemcpyDeviceToHost) != hipSuccess)
-        rc = MR_ERR_HIP;
+      rc = queue_readback(f);
+      if (!rc && hipStreamSynchronize(f.slot.stream) != hipSuccess) rc = MR_ERR_HIP;
     }
   }
-  static const size_t kBytes[5] = {3, 4, 4, 12, 12};
   for (size_t j = 0; j < f.reqs.size() && rc == MR_OK; j++) {
     const mr_outputs& o = f.reqs[j]->out;
     if (o.on_device) continue;
     const size_t npix = (size_t)f.plan.jobs[j].W * f.plan.jobs[j].H;
     void* dst[5] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals};
     for (int k = 0; k < 5; k++)
-      if (dst[k])
-        memcpy(dst[k], (const char*)f.slot.sc.h_outputs.p + f.offs[j * 5 + k], npix * kBytes[k]);
+      if (dst[k] && f.staged[j * 5 + k])
+        memcpy(dst[k], (const char*)f.slot.sc.h_outputs.p + f.offs[j * 5 + k],
+               npix * kOutBytes[k]);
   }
   std::lock_guard<std::mutex> lk(mu);
   const double t = now_us();

@@ -85,3 +85,23 @@ def fuzz_frame(seed: int, width: int = 97, height: int = 61) -> FrameSpec:
                      background=tuple(rng.uniform(0, 1, 3)), ambient=float(rng.uniform(0, 0.3)),
                      lighting=bool(rng.random() < 0.85), texturing=bool(rng.random() < 0.8),
                      save_normals=bool(rng.random() < 0.5))
+
+
+def stack_frame(layers: int = 1500, width: int = 80, height: int = 56, seed: int = 0) -> FrameSpec:
+    """`layers` large screen-covering triangles at random depths (some equal,
+    for ties), so every tile holds more (tile, triangle) pairs than one
+    raster work item and is split into parts merged through global memory."""
+    rng = np.random.default_rng(seed)
+    z = -rng.choice(np.linspace(2.0, 6.0, 97).astype(np.float32), layers)
+    jit = rng.uniform(-0.3, 0.3, (layers, 3, 2)).astype(np.float32)
+    base = np.array([[-3.0, -3.0], [3.5, -2.5], [-0.5, 3.5]], np.float32)
+    pos = np.empty((layers, 3, 3), np.float32)
+    pos[:, :, :2] = (base[None] + jit) * (-z[:, None, None] / 2.0)
+    pos[:, :, 2] = z[:, None]
+    nrm = np.tile(np.array([0.0, 0.0, 1.0], np.float32), (layers * 3, 1))
+    idx = np.arange(layers * 3, dtype=np.int32).reshape(-1, 3)
+    m = scenes.Mesh(pos.reshape(-1, 3), nrm, idx)
+    proj = scenes.projection_frustum(math.radians(60.0), width / height, 0.5, 20.0)
+    mat = Material(diffuse=(0.6, 0.5, 0.4), specular=(0.3, 0.3, 0.3), shininess=8.0)
+    return FrameSpec(width, height, proj, draws=[Draw(m, mat)], light=(1.0, 2.0, 3.0),
+                     light_is_point=True)

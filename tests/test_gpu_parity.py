@@ -13,7 +13,7 @@ import pytest
 from minirender_amd import scenes
 from minirender_amd.scenes import Draw, FrameSpec, Material
 from oracle import oracle_py
-from tests.parity import assert_parity, fuzz_frame
+from tests.parity import assert_parity, fuzz_frame, stack_frame
 
 pytestmark = pytest.mark.gpu
 
@@ -52,12 +52,16 @@ def test_fuzz_scenes(gpu_ctx, seed):
     check(gpu_ctx, fuzz_frame(seed), f"fuzz seed {seed}")
 
 
-@pytest.mark.parametrize("tile", [8, 16, 32])
-def test_tile_sizes(gpu_ctx, tile):
+@pytest.mark.parametrize("tile,threads", [(8, 64), (8, 128), (16, 128), (16, 256), (16, 512),
+                                          (32, 256), (32, 512)])
+def test_tile_sizes(gpu_ctx, tile, threads):
     gpu_ctx.set_tile(tile, tile)
+    gpu_ctx.set_raster_threads(threads)
     try:
         for seed in (100, 101, 102):
-            check(gpu_ctx, fuzz_frame(seed, 133, 71), f"tile {tile} seed {seed}")
+            check(gpu_ctx, fuzz_frame(seed, 133, 71), f"tile {tile}/{threads} seed {seed}")
+        # split tiles (more pairs than one raster work item) with this shape
+        check(gpu_ctx, stack_frame(1500, 80, 56, seed=tile + threads), f"tile {tile}/{threads} split")
     finally:
         gpu_ctx.set_tile(16, 16)
 
