@@ -157,24 +157,6 @@ def device_outputs(frame, torch, dev):
     return o, (rgb8, depth)
 
 
-def autotune_tile(ctx, run, tiles=(8, 16, 32), reps=3):
-    """Pick the tile size with the lowest median time of `run()`."""
-    best, table = None, {}
-    for t in tiles:
-        ctx.set_tile(t, t)
-        run()  # warm scratch for this tile size
-        ts = []
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            run()
-            ts.append(time.perf_counter() - t0)
-        table[t] = float(np.median(ts))
-        if best is None or table[t] < table[best]:
-            best = t
-    ctx.set_tile(best, best)
-    return best, table
-
-
 def plain_section(args, torch, mra, frame, wl_name):
     dev = torch.device("cuda:0")
     ctx = mra.Context(0)
@@ -187,11 +169,12 @@ def plain_section(args, torch, mra, frame, wl_name):
             raise RuntimeError(f"mr_render failed: {rc}")
 
     if args.tile:
-        tile, table = args.tile, None
-        ctx.set_tile(tile, tile)
+        ctx.set_tile(args.tile, args.tile)
+        tune = {"tile": args.tile, "fixed": True}
     else:
-        tile, table = autotune_tile(ctx, step)
-    log(f"plain: tile {tile} (autotune {table})")
+        tune = ctx.autotune([frame], outs=[outs], reps=3)
+    tile = tune["tile"]
+    log(f"plain: {tune}")
     for _ in range(args.warmup):
         step()
     ctx.set_profiling(True)
@@ -230,7 +213,7 @@ def plain_section(args, torch, mra, frame, wl_name):
                       "achieved": round(fb / (fmed * 1e-3) / 1e9, 1),
                       "frac": round(fb / (fmed * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
     res = {"ms_per_step": dt * 1e3, "value": frame.ntri / dt / 1e6, "frames_per_s": 1.0 / dt,
-           "tile": tile, "autotune_s": table, "kernels": kernels, "roofline": roof,
+           "tile": tile, "autotune": tune, "kernels": kernels, "roofline": roof,
            "records": st.records, "bin_entries": st.bin_entries,
            "clipped": st.triangles_clipped, "retries": st.retries}
     ctx.close()
@@ -287,7 +270,8 @@ def batched_section(args, torch, mra):
             if rc:
                 raise RuntimeError(f"mr_render failed: {rc}")
 
-    tile, table = autotune_tile(ctx, batch)
+    tune = ctx.autotune(jobs, outs=outs, reps=2)
+    log(f"batched: {tune}")
     reps = max(3, args.steps // 5)
     batch()
     torch.cuda.synchronize()
@@ -303,7 +287,7 @@ def batched_section(args, torch, mra):
     torch.cuda.synchronize()
     tp = (time.perf_counter() - t0) / max(1, reps // 2)
     ctx.close()
-    return {"jobs": n, "triangles": tris, "tile": tile, "autotune_s": table,
+    return {"jobs": n, "triangles": tris, "autotune": tune,
             "batched": {"ms_per_pass": tb * 1e3, "frames_per_s": n / tb, "Mtris_per_s": tris / tb / 1e6},
             "plain": {"ms_per_pass": tp * 1e3, "frames_per_s": n / tp, "Mtris_per_s": tris / tp / 1e6},
             "speedup": tp / tb}
@@ -395,7 +379,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded generators)",
         "config": {"workload": wl, "triangles": frame.ntri, "width": frame.width,
-                   "height": frame.height, "tile": plain["tile"], "path": "plain mr_render",
+                   "height": frame.height, "tile": plain["tile"],
+                   "autotune": plain["autotune"], "path": "plain mr_render",
                    "outputs": "device (RGB8 + fp32 depth)"},
         "roofline": plain["roofline"],
         "kernels": plain["kernels"],

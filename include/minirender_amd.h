@@ -119,6 +119,27 @@ int mr_ctx_set_tile(mr_ctx* ctx, int32_t tile_w, int32_t tile_h);
  * default; 8x8: 64 or 128, 16x16: 128, 256 or 512, 32x32: 256 or 512).
  * mr_ctx_set_tile resets it to the default. */
 int mr_ctx_set_raster_threads(mr_ctx* ctx, int32_t threads);
+/* Autotuner (no counterpart in the reference).  Renders `frames` (n >= 1,
+ * with outputs `outs`) under every built combination of tile size (8, 16,
+ * 32), raster workgroup size and split size of crowded tiles, `reps` timed
+ * runs each (HIP events on the context's stream), keeps the fastest and
+ * sets the bin-list capacity to the measured pairs per triangle + 25%.  The
+ * choice is applied to the context and returned in *best.  With cache_path,
+ * a previous result for the same GPU and batch shape (frame count,
+ * triangles, pixels, rounded up to powers of two) is read from that text
+ * file instead of measuring, and new results are appended to it. */
+typedef struct mr_tuning {
+  int32_t tile;            /* tile edge in pixels                              */
+  int32_t raster_threads;  /* raster workgroup size                            */
+  float bin_factor;        /* bin-list entries per input triangle              */
+  int32_t frag_size;       /* max (tile, triangle) pairs per raster work item  */
+  double ms;               /* median time of one call with this choice         */
+  int32_t from_cache;      /* 1 if read from cache_path                        */
+  int32_t candidates;      /* configurations timed                             */
+} mr_tuning;
+int mr_ctx_autotune(mr_ctx* ctx, const mr_frame* frames, int32_t nframes,
+                    const struct mr_outputs* outs, int32_t reps,
+                    const char* cache_path, mr_tuning* best);
 /* Bin-list capacity in (tile, triangle) entries per input triangle
  * (>= 1); overflow re-runs the frame with a larger list. */
 int mr_ctx_set_bin_factor(mr_ctx* ctx, float entries_per_triangle);

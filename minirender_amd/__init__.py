@@ -205,6 +205,32 @@ class Context:
         return outs
 
 
+    def autotune(self, frames: List[FrameSpec], outs=None, reps: int = 3,
+                 cache_path: Optional[str] = None) -> dict:
+        """Time the launch configurations on `frames` (outputs: `outs`, a list
+        of mr_outputs structs, or host RGB8 + depth buffers) and apply the
+        fastest; see mr_ctx_autotune."""
+        from ._capi import mr_tuning
+        structs, keep = [], []
+        for fr in frames:
+            f, k = self.frame(fr)
+            structs.append(f)
+            keep.append(k)
+        host = None
+        if outs is None:
+            host = [alloc_outputs(fr, ("rgb8", "depth")) for fr in frames]
+            outs = [outputs_struct(o) for o in host]
+        farr = (mr_frame * len(frames))(*structs)
+        oarr = (mr_outputs * len(frames))(*outs)
+        t = mr_tuning()
+        check(self.lib.mr_ctx_autotune(self.h, farr, len(frames), oarr, reps,
+                                       cache_path.encode() if cache_path else None, C.byref(t)),
+              "mr_ctx_autotune")
+        return {"tile": t.tile, "raster_threads": t.raster_threads,
+                "bin_factor": round(t.bin_factor, 3), "frag_size": t.frag_size,
+                "ms": t.ms, "from_cache": bool(t.from_cache), "candidates": t.candidates}
+
+
 class Server:
     """Render service with continuous batching over a Context's resident
     meshes (C-ABI mr_server_*).  max_batch=1, slots=1 is the plain

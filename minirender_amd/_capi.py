@@ -63,6 +63,14 @@ class mr_stats(C.Structure):
     ]
 
 
+class mr_tuning(C.Structure):
+    _fields_ = [
+        ("tile", C.c_int32), ("raster_threads", C.c_int32), ("bin_factor", C.c_float),
+        ("frag_size", C.c_int32), ("ms", C.c_double), ("from_cache", C.c_int32),
+        ("candidates", C.c_int32),
+    ]
+
+
 class mr_server_config(C.Structure):
     _fields_ = [
         ("max_batch", C.c_int32), ("slots", C.c_int32),
@@ -104,6 +112,8 @@ SIGNATURES = {
     "mr_texture_release": (C.c_int, [_P, _I]),
     "mr_render": (C.c_int, [_P, C.POINTER(mr_frame), C.POINTER(mr_outputs)]),
     "mr_render_batch": (C.c_int, [_P, C.POINTER(mr_frame), _I, C.POINTER(mr_outputs)]),
+    "mr_ctx_autotune": (C.c_int, [_P, C.POINTER(mr_frame), _I, C.POINTER(mr_outputs), _I,
+                                  C.c_char_p, C.POINTER(mr_tuning)]),
     "mr_server_create": (C.c_int, [_P, C.POINTER(mr_server_config), C.POINTER(_P)]),
     "mr_server_destroy": (C.c_int, [_P]),
     "mr_server_submit": (C.c_int, [_P, C.POINTER(mr_frame), C.POINTER(mr_outputs),

@@ -167,3 +167,24 @@ def test_graph_replay_matches_eager(gpu_ctx):
     finally:
         ctx.close()
     assert_parity(eager[2], oracle_py.render(views[2], ALL), "graph view 2", normals=False)
+
+
+def test_autotune_and_cache(tmp_path):
+    """The autotuner picks a built configuration, results stay exact under
+    it, and a second call with the same cache file reads the choice back."""
+    import minirender_amd as mra
+    frames = [fuzz_frame(300 + i, 160, 120) for i in range(3)]
+    cache = str(tmp_path / "tune.txt")
+    ctx = mra.Context(0)
+    try:
+        t1 = ctx.autotune(frames, reps=1, cache_path=cache)
+        assert t1["tile"] in (8, 16, 32) and not t1["from_cache"] and t1["candidates"] >= 10
+        assert t1["bin_factor"] >= 1.0 and t1["ms"] > 0
+        for i, fr in enumerate(frames):
+            check(ctx, fr, f"autotuned frame {i}")
+        t2 = ctx.autotune(frames, reps=1, cache_path=cache)
+        assert t2["from_cache"]
+        for k in ("tile", "raster_threads", "frag_size"):
+            assert t2[k] == t1[k]
+    finally:
+        ctx.close()
