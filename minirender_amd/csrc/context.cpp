@@ -114,7 +114,8 @@ void slot_free(mr_ctx* c, Slot& s) {
   HostBuf* hbufs[] = {&x.h_draws, &x.h_draw_begin, &x.h_jobs, &x.h_job_bases,
                       &x.h_counters, &x.h_outputs};
   for (HostBuf* b : hbufs) host_free(*b);
-  if (s.graph) hipGraphExecDestroy(s.graph);
+  for (auto& g : s.graphs)
+    if (g.exec) hipGraphExecDestroy(g.exec);
   for (auto& e : s.ev)
     if (e) hipEventDestroy(e);
   if (s.done) hipEventDestroy(s.done);
@@ -368,12 +369,13 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
     std::vector<unsigned char> key(sizeof(BatchArgs) + sizeof(hp));
     memcpy(key.data(), &a, sizeof(BatchArgs));
     memcpy(key.data() + sizeof(BatchArgs), hp, sizeof(hp));
-    if (!s.graph || key != s.graph_key) {
-      if (s.graph) {
-        hipGraphExecDestroy(s.graph);
-        s.graph = nullptr;
+    size_t hit = s.graphs.size();
+    for (size_t i = 0; i < s.graphs.size(); i++)
+      if (s.graphs[i].key == key) {
+        hit = i;
+        break;
       }
-      s.graph_key.clear();
+    if (hit == s.graphs.size()) {
       HIP_TRY(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
       const int r = issue();
       hipGraph_t g = nullptr;
@@ -383,17 +385,22 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
         return r;
       }
       if (e != hipSuccess) return MR_ERR_HIP;
-      const hipError_t ei = hipGraphInstantiate(&s.graph, g, nullptr, nullptr, 0);
+      Slot::Graph ng;
+      const hipError_t ei = hipGraphInstantiate(&ng.exec, g, nullptr, nullptr, 0);
       hipGraphDestroy(g);
-      if (ei != hipSuccess) {
-        s.graph = nullptr;
-        return MR_ERR_HIP;
+      if (ei != hipSuccess) return MR_ERR_HIP;
+      ng.key = std::move(key);
+      if (s.graphs.size() >= kGraphCache) {
+        hipGraphExecDestroy(s.graphs.back().exec);
+        s.graphs.pop_back();
       }
-      s.graph_key = std::move(key);
+      s.graphs.insert(s.graphs.begin(), std::move(ng));
+      s.graph_captures++;
     } else {
+      if (hit) std::rotate(s.graphs.begin(), s.graphs.begin() + hit, s.graphs.begin() + hit + 1);
       s.graph_replays++;
     }
-    HIP_TRY(hipGraphLaunch(s.graph, s.stream));
+    HIP_TRY(hipGraphLaunch(s.graphs.front().exec, s.stream));
   } else {
     if ((rc = issue())) return rc;
   }

@@ -12,6 +12,8 @@
 
 namespace mr {
 
+constexpr size_t kGraphCache = 32;
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -68,11 +70,15 @@ struct Slot {
   uint32_t heavy_slots = 0;  // merge buffers of split tiles (grown on demand)
   int32_t last_geom_grid = 0, last_bin_grid = 0;
   int64_t retries = 0;
-  // Steady state: the last batch's uploads + stages + counter read-back as
-  // an instantiated graph, replayed while the launch key is unchanged.
-  hipGraphExec_t graph = nullptr;
-  std::vector<unsigned char> graph_key;
-  int64_t graph_replays = 0;
+  // Steady state: batches' uploads + stages + counter read-back captured as
+  // instantiated graphs, keyed by the launch parameters, most recent first
+  // (at most kGraphCache); a batch with a known key replays its graph.
+  struct Graph {
+    std::vector<unsigned char> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<Graph> graphs;
+  int64_t graph_replays = 0, graph_captures = 0;
 };
 
 }  // namespace mr
