@@ -155,7 +155,8 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
     const int64_t npix = (int64_t)f->width * f->height;
     J.shade_base = (int32_t)plan.shade_blocks;
     plan.job_bases[nframes + j] = J.shade_base;
-    plan.shade_blocks += (npix + kShadeBlock - 1) / kShadeBlock;
+    plan.shade_blocks += (int64_t)((f->width + kShadeW - 1) / kShadeW) *
+                         ((f->height + kShadeH - 1) / kShadeH);
     plan.npixels += npix;
     if (plan.shade_blocks > 0x7FFFFFFF) return MR_ERR_ARG;
     J.draw_begin = (int32_t)plan.draws.size();
