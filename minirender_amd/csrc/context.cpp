@@ -436,8 +436,8 @@ int finish_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   if (&s == &c->main) {
     mr_stats& st = c->stats;
     st.triangles_in = p.total_tris;
-    st.triangles_clipped = -1;  // summed from block_stats on request
-    st.records = -1;
+    st.triangles_clipped = (int64_t)hc[kClipped];
+    st.records = (int64_t)hc[kItems];
     st.bin_entries = (int64_t)hc[kBinEntries];
     st.tiles = p.ntiles;
     st.retries = s.retries;
@@ -637,18 +637,6 @@ int mr_ctx_set_graphs(mr_ctx* c, int32_t on) {
 int mr_ctx_get_stats(mr_ctx* c, mr_stats* st) {
   if (!c || !st) return MR_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
-  Slot& s = c->main;
-  if (c->stats.records < 0 && s.sc.block_stats.p) {
-    // Sum the per-workgroup statistics of the last batch.
-    std::vector<unsigned long long> h(2 * kStatSlots);
-    HIP_TRY(hipStreamSynchronize(s.stream));
-    HIP_TRY(hipMemcpy(h.data(), s.sc.block_stats.p, h.size() * 8, hipMemcpyDeviceToHost));
-    int64_t clipped = 0, records = 0;
-    for (int i = 0; i < s.last_geom_grid; i++) clipped += (int64_t)h[i];
-    for (int i = 0; i < s.last_bin_grid; i++) records += (int64_t)h[kStatSlots + i];
-    c->stats.triangles_clipped = clipped;
-    c->stats.records = records;
-  }
   *st = c->stats;
   return MR_OK;
 }

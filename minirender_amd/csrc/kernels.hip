@@ -664,6 +664,36 @@ __device__ void shade_record(const BatchArgs& a, const JobDesc& J, ShadeLds& L, 
   }
 }
 
+// Statistics of the frame: sum the per-workgroup counts of the geometry and
+// bin kernels (finished before shading starts) into the counters read back
+// with the batch.  Run by one shading workgroup; uses one LDS word per wave.
+__device__ void reduce_stats(const BatchArgs& a, int tid) {
+  __shared__ unsigned long long part[2][kShadeBlock / 64];
+  const int ng = a.total_tris ? geom_grid(a) : 0, nb = bin_grid(a);
+  unsigned long long c = 0, r = 0;
+  for (int i = tid; i < ng; i += kShadeBlock) c += a.block_stats[i];
+  for (int i = tid; i < nb; i += kShadeBlock) r += a.block_stats[kStatSlots + i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    c += __shfl_xor(c, o);
+    r += __shfl_xor(r, o);
+  }
+  if ((tid & 63) == 0) {
+    part[0][tid >> 6] = c;
+    part[1][tid >> 6] = r;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long sc = 0, sr = 0;
+    for (int w = 0; w < kShadeBlock / 64; w++) {
+      sc += part[0][w];
+      sr += part[1][w];
+    }
+    a.counters[kClipped] = sc;
+    a.counters[kItems] = sr;
+  }
+}
+
 __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
   __shared__ ShadeLds L;
   const int tid = threadIdx.x;
@@ -701,6 +731,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
   }
   __syncthreads();
   const int nuniq = L.count;
+  if (blockIdx.x == 0) reduce_stats(a, tid);
   const bool attrs = J.rgb8 || J.rgb || (J.normals && J.save_normals);
   for (int u = tid; u < nuniq; u += kShadeBlock) shade_record(a, J, L, ncached, u, xb0, yb0, attrs);
   __syncthreads();

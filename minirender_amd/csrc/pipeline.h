@@ -53,12 +53,12 @@ inline int32_t scan_blocks_for(int32_t tiles) {
 }
 
 // Grid sizes of the grid-stride kernels (also used to sum their statistics).
-inline int32_t geom_grid(const BatchArgs& a) {
+__host__ __device__ inline int32_t geom_grid(const BatchArgs& a) {
   const int64_t blocks = (a.total_tris + 255) / 256;
   const int64_t cap = (int64_t)a.num_cus * 8;
   return (int32_t)(blocks < cap ? blocks : cap);
 }
-inline int32_t bin_grid(const BatchArgs& a) {
+__host__ __device__ inline int32_t bin_grid(const BatchArgs& a) {
   int64_t blocks = (a.nregions + 3) / 4;
   if (blocks < 1) blocks = 1;
   const int64_t cap = (int64_t)a.num_cus * 8;
