@@ -109,10 +109,9 @@ void slot_free(mr_ctx* c, Slot& s) {
   DevBuf* bufs[] = {&x.items, &x.region_count, &x.tile_count, &x.tile_off,
                     &x.tile_cursor, &x.tile_heavy, &x.tile_done, &x.parts, &x.bins,
                     &x.heavy_keys, &x.scan_status, &x.counters, &x.block_stats,
-                    &x.draws, &x.draw_begin, &x.jobs, &x.job_bases, &x.win, &x.outputs};
+                    &x.tables, &x.win, &x.outputs};
   for (DevBuf* b : bufs) dev_free(c, *b, true);
-  HostBuf* hbufs[] = {&x.h_draws, &x.h_draw_begin, &x.h_jobs, &x.h_job_bases,
-                      &x.h_counters, &x.h_outputs};
+  HostBuf* hbufs[] = {&x.h_tables, &x.h_counters, &x.h_outputs};
   for (HostBuf* b : hbufs) host_free(*b);
   for (auto& g : s.graphs)
     if (g.exec) hipGraphExecDestroy(g.exec);
@@ -217,6 +216,14 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
     J.draw_end = (int32_t)plan.draws.size();
   }
   if (2 * plan.total_tris >= 0xFFFFFFFFll) return MR_ERR_ARG;
+  auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  TableLayout& L = plan.layout;
+  const size_t nd = std::max<size_t>(plan.draws.size(), 1);
+  L.jobs = 0;
+  L.bases = up(plan.jobs.size() * sizeof(JobDesc));
+  L.draws = L.bases + up(plan.job_bases.size() * 4);
+  L.draw_begin = L.draws + up(nd * sizeof(DrawDesc));
+  L.bytes = L.draw_begin + up(nd * 8);
   return MR_OK;
 }
 
@@ -226,7 +233,6 @@ static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins
   const int64_t T = p.total_tris;
   const int scb = scan_blocks_for((int32_t)p.ntiles);
   const size_t nt = (size_t)p.ntiles * 4;
-  const size_t nd = std::max<size_t>(p.draws.size(), 1);
   const int64_t nregions = std::max<int64_t>((T + 63) / 64, 1);
   const size_t npt = (size_t)c->tile_w * c->tile_h * 8;
   const size_t parts = (size_t)(bins_cap / c->frag_size) + heavy_slots + 16;
@@ -244,15 +250,9 @@ static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins
   if ((rc = dev_alloc(c, s.scan_status, (size_t)std::max(scb, 1) * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.counters, kNumCounters * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.block_stats, 2 * kStatSlots * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.draws, nd * sizeof(DrawDesc), true))) return rc;
-  if ((rc = dev_alloc(c, s.draw_begin, nd * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.jobs, p.jobs.size() * sizeof(JobDesc), true))) return rc;
-  if ((rc = dev_alloc(c, s.job_bases, p.job_bases.size() * 4, true))) return rc;
+  if ((rc = dev_alloc(c, s.tables, p.layout.bytes, true))) return rc;
   if ((rc = dev_alloc(c, s.win, (size_t)p.npixels * 4, true))) return rc;
-  if ((rc = host_alloc(s.h_draws, nd * sizeof(DrawDesc)))) return rc;
-  if ((rc = host_alloc(s.h_draw_begin, nd * 8))) return rc;
-  if ((rc = host_alloc(s.h_jobs, p.jobs.size() * sizeof(JobDesc)))) return rc;
-  if ((rc = host_alloc(s.h_job_bases, p.job_bases.size() * 4))) return rc;
+  if ((rc = host_alloc(s.h_tables, p.layout.bytes))) return rc;
   if ((rc = host_alloc(s.h_counters, kNumCounters * 8))) return rc;
   return MR_OK;
 }
@@ -280,12 +280,13 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.scan_status = (unsigned long long*)s.scan_status.p;
   a.counters = (unsigned long long*)s.counters.p;
   a.block_stats = (unsigned long long*)s.block_stats.p;
-  a.draws = (const DrawDesc*)s.draws.p;
-  a.draw_begin = (const int64_t*)s.draw_begin.p;
+  char* tb = (char*)s.tables.p;
+  a.draws = (const DrawDesc*)(tb + p.layout.draws);
+  a.draw_begin = (const int64_t*)(tb + p.layout.draw_begin);
   a.ndraws = (int32_t)p.draws.size();
-  a.jobs = (const JobDesc*)s.jobs.p;
-  a.job_tile_base = (const int32_t*)s.job_bases.p;
-  a.job_shade_base = (const int32_t*)s.job_bases.p + p.jobs.size();
+  a.jobs = (const JobDesc*)(tb + p.layout.jobs);
+  a.job_tile_base = (const int32_t*)(tb + p.layout.bases);
+  a.job_shade_base = (const int32_t*)(tb + p.layout.bases) + p.jobs.size();
   a.njobs = (int32_t)p.jobs.size();
   a.total_tris = p.total_tris;
   a.ntiles = (int32_t)p.ntiles;
@@ -336,26 +337,19 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   }
   // Host tables go to pinned staging first; the uploads read them from there
   // (also on a graph replay, which re-reads the same staging).
-  memcpy(x.h_jobs.p, p.jobs.data(), p.jobs.size() * sizeof(JobDesc));
-  memcpy(x.h_job_bases.p, p.job_bases.data(), p.job_bases.size() * 4);
+  char* ht = (char*)x.h_tables.p;
+  memcpy(ht + p.layout.jobs, p.jobs.data(), p.jobs.size() * sizeof(JobDesc));
+  memcpy(ht + p.layout.bases, p.job_bases.data(), p.job_bases.size() * 4);
   if (!p.draws.empty()) {
-    memcpy(x.h_draws.p, p.draws.data(), p.draws.size() * sizeof(DrawDesc));
-    memcpy(x.h_draw_begin.p, p.draw_begin.data(), p.draws.size() * 8);
+    memcpy(ht + p.layout.draws, p.draws.data(), p.draws.size() * sizeof(DrawDesc));
+    memcpy(ht + p.layout.draw_begin, p.draw_begin.data(), p.draws.size() * 8);
   }
   const BatchArgs a = make_args(c, s, p);
   s.last_geom_grid = p.total_tris ? geom_grid(a) : 0;
   s.last_bin_grid = bin_grid(a);
   auto issue = [&]() -> int {
-    HIP_TRY(hipMemcpyAsync(x.jobs.p, x.h_jobs.p, p.jobs.size() * sizeof(JobDesc),
-                           hipMemcpyHostToDevice, s.stream));
-    HIP_TRY(hipMemcpyAsync(x.job_bases.p, x.h_job_bases.p, p.job_bases.size() * 4,
-                           hipMemcpyHostToDevice, s.stream));
-    if (!p.draws.empty()) {
-      HIP_TRY(hipMemcpyAsync(x.draws.p, x.h_draws.p, p.draws.size() * sizeof(DrawDesc),
-                             hipMemcpyHostToDevice, s.stream));
-      HIP_TRY(hipMemcpyAsync(x.draw_begin.p, x.h_draw_begin.p, p.draws.size() * 8,
-                             hipMemcpyHostToDevice, s.stream));
-    }
+    HIP_TRY(hipMemcpyAsync(x.tables.p, x.h_tables.p, p.layout.bytes, hipMemcpyHostToDevice,
+                           s.stream));
     int r = launch_stages(a, s, profile);
     if (r) return r;
     HIP_TRY(hipMemcpyAsync(x.h_counters.p, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost,
@@ -365,8 +359,7 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   // Graphs only on the slot's own stream (a caller's stream may be the null
   // stream, which cannot be captured) and without per-stage events.
   if (c->graphs && !profile && s.own_stream && s.stream == s.own_stream) {
-    const void* hp[5] = {x.h_jobs.p, x.h_job_bases.p, x.h_draws.p, x.h_draw_begin.p,
-                         x.h_counters.p};
+    const void* hp[3] = {x.h_tables.p, x.h_counters.p, (const void*)p.layout.bytes};
     std::vector<unsigned char> key(sizeof(BatchArgs) + sizeof(hp));
     memcpy(key.data(), &a, sizeof(BatchArgs));
     memcpy(key.data() + sizeof(BatchArgs), hp, sizeof(hp));

@@ -44,8 +44,15 @@ struct TexRec {
 struct Scratch {
   DevBuf items, region_count, tile_count, tile_off, tile_cursor, tile_heavy,
       tile_done, parts, bins, heavy_keys, scan_status, counters, block_stats,
-      draws, draw_begin, jobs, job_bases, win, outputs;
-  HostBuf h_draws, h_draw_begin, h_jobs, h_job_bases, h_counters, h_outputs;
+      tables, win, outputs;
+  // The batch tables (jobs, job tile/shade bases, draws, draw starts) sit in
+  // one pinned staging buffer and one device buffer, uploaded by one copy.
+  HostBuf h_tables, h_counters, h_outputs;
+};
+
+// Byte offsets of the tables inside Scratch::tables.
+struct TableLayout {
+  size_t jobs = 0, bases = 0, draws = 0, draw_begin = 0, bytes = 0;
 };
 
 // Jobs and draws of a batch, as the kernels see them.
@@ -58,6 +65,7 @@ struct BatchPlan {
   int64_t ntiles = 0;
   int64_t npixels = 0;
   int64_t shade_blocks = 0;
+  TableLayout layout;
 };
 
 // An execution slot: scratch, stream and events for one batch in flight.
