@@ -549,82 +549,53 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
   }
 }
 
-// (e) Deferred shading (Renderer.cpp:271-305).  A workgroup shades a
-// kShadeW x kShadeH block of one job, one lane per pixel.  The block's
-// winners are deduplicated through an LDS hash table; one lane per distinct
-// triangle rebuilds it once (setup, eye-space vertices after clipping) and
-// runs the edge recurrence of every block row up to the block's first column
-// inside the triangle's bbox.  A pixel then continues its row's recurrence
-// for fewer than kShadeW steps, interpolates and lights.
-constexpr int kHashSlots = 2 * kShadeBlock;
-constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // == kNoWinner, never a triangle id
-constexpr int kAttrs = 24;                // p0 p1 p2, n0 n1 n2 (xyz), uv0 uv1 uv2
-
-struct ShadeLds {
-  uint32_t key[kHashSlots];
-  uint32_t val[kHashSlots];
-  uint32_t uniq[kShadeBlock];
-  float n1x[kShadeBlock], n2x[kShadeBlock], z0[kShadeBlock], z1[kShadeBlock], z2[kShadeBlock];
-  int cst[kShadeBlock], draw[kShadeBlock];
-  float e1[kShadeH][kShadeBlock], e2[kShadeH][kShadeBlock];
-  float attr[kAttrs][kShadeBlock];
-  int64_t dcache[kMaxLdsDraws];
-  int count;
-};
-
-__device__ inline void store_background(const JobDesc& J, size_t pix) {
-  if (J.rgb8) {
-    J.rgb8[pix * 3 + 0] = quantize(J.bg[0]);
-    J.rgb8[pix * 3 + 1] = quantize(J.bg[1]);
-    J.rgb8[pix * 3 + 2] = quantize(J.bg[2]);
+// Deferred shading of one pixel from its winner (Renderer.cpp:271-305).
+__device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
+                            const int64_t* dcache, int ncached, uint32_t id,
+                            int c, int row) {
+  const size_t pix = (size_t)row * (size_t)J.W + (size_t)c;
+  if (id == kNoWinner) {  // clear(), Renderer.cpp:113-119
+    if (J.rgb8) {
+      J.rgb8[pix * 3 + 0] = quantize(J.bg[0]);
+      J.rgb8[pix * 3 + 1] = quantize(J.bg[1]);
+      J.rgb8[pix * 3 + 2] = quantize(J.bg[2]);
+    }
+    if (J.depth) J.depth[pix] = kClearDepth;
+    if (J.winner) J.winner[pix] = kNoWinner;
+    if (J.rgb) {
+      J.rgb[pix * 3 + 0] = J.bg[0];
+      J.rgb[pix * 3 + 1] = J.bg[1];
+      J.rgb[pix * 3 + 2] = J.bg[2];
+    }
+    if (J.normals && J.save_normals) {
+      J.normals[pix * 3 + 0] = 0.0f;
+      J.normals[pix * 3 + 1] = 0.0f;
+      J.normals[pix * 3 + 2] = 1.0f;
+    }
+    return;
   }
-  if (J.depth) J.depth[pix] = kClearDepth;  // clear(), Renderer.cpp:113-119
-  if (J.winner) J.winner[pix] = kNoWinner;
-  if (J.rgb) {
-    J.rgb[pix * 3 + 0] = J.bg[0];
-    J.rgb[pix * 3 + 1] = J.bg[1];
-    J.rgb[pix * 3 + 2] = J.bg[2];
-  }
-  if (J.normals && J.save_normals) {
-    J.normals[pix * 3 + 0] = 0.0f;
-    J.normals[pix * 3 + 1] = 0.0f;
-    J.normals[pix * 3 + 2] = 1.0f;
-  }
-}
-
-// Rebuild distinct triangle u of the block: setup, row starts and, when
-// colour or normals are written, the eye-space vertices of the (clipped)
-// triangle as paintMesh / clipTriangle produce them (Renderer.cpp:344-379,
-// 131-161).
-__device__ void shade_record(const BatchArgs& a, const JobDesc& J, ShadeLds& L, int ncached,
-                             int u, int xb0, int yb0, bool attrs) {
-  const uint32_t id = L.uniq[u];
   const int64_t g = id >> 1;
   const int sub = (int)(id & 1u);
-  const int d = lookup_draw(L.dcache, ncached, a, J, g);
-  const DrawDesc& D = a.draws[d];
+  const DrawDesc& D = a.draws[lookup_draw(dcache, ncached, a, J, g)];
   const int tri = (int)(g - D.tri_begin);
   TriSetup s;
   tri_setup_sub(D, J, tri, sub, s);
-  const int cst = max(s.bx0, xb0);
-  L.n1x[u] = s.n1x;
-  L.n2x[u] = s.n2x;
-  L.z0[u] = s.z0;
-  L.z1[u] = s.z1;
-  L.z2[u] = s.z2;
-  L.cst[u] = cst;
-  L.draw[u] = d;
-#pragma unroll
-  for (int r = 0; r < kShadeH; r++) {
-    const int row = yb0 + r;
-    if (row < s.by0 || row > s.by1) continue;  // no pixel of this row can be its
-    float e1, e2;
-    edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
-               (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
-    L.e1[r][u] = advance(e1, s.n1x, cst - s.bx0);
-    L.e2[r][u] = advance(e2, s.n2x, cst - s.bx0);
+  float e1, e2;
+  edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
+             (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
+  const int n = c - s.bx0;
+  e1 = advance(e1, s.n1x, n);
+  e2 = advance(e2, s.n2x, n);
+  float k[3];
+  const float z = depth_of(e1, e2, s.z0, s.z1, s.z2, J.persp != 0, k);
+  if (!J.rgb8 && !J.rgb && !(J.normals && J.save_normals)) {
+    if (J.depth) J.depth[pix] = z;
+    if (J.winner) J.winner[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
+    return;
   }
-  if (!attrs) return;
+
+  // The triangle's eye-space vertices, recomputed as paintMesh does
+  // (Renderer.cpp:344-379), and re-clipped for a clipped sub-triangle.
   Vtx v0, v1, v2;
   {
     const int* ip = D.idx + 3 * tri;
@@ -650,18 +621,53 @@ __device__ void shade_record(const BatchArgs& a, const JobDesc& J, ShadeLds& L, 
     if (sub) { v0 = b0; v1 = b1; v2 = b2; }
     else { v0 = a0; v1 = a1; v2 = a2; }
   }
-  const Vtx* v[3] = {&v0, &v1, &v2};
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    L.attr[3 * i + 0][u] = v[i]->p.x;
-    L.attr[3 * i + 1][u] = v[i]->p.y;
-    L.attr[3 * i + 2][u] = v[i]->p.z;
-    L.attr[9 + 3 * i + 0][u] = v[i]->n.x;
-    L.attr[9 + 3 * i + 1][u] = v[i]->n.y;
-    L.attr[9 + 3 * i + 2][u] = v[i]->n.z;
-    L.attr[18 + 2 * i + 0][u] = v[i]->uv.x;
-    L.attr[18 + 2 * i + 1][u] = v[i]->uv.y;
+
+  f3 color = f3{D.diffuse[0], D.diffuse[1], D.diffuse[2]};
+  if (D.tex) {  // :271-275
+    const float u = (k[0] * v0.uv.x + k[1] * v1.uv.x) + k[2] * v2.uv.x;
+    const float v = (k[0] * v0.uv.y + k[1] * v1.uv.y) + k[2] * v2.uv.y;
+    color = texel(D.tex, D.tex_format, D.tex_cols, tex_index(v, D.tex_rows),
+                  tex_index(u, D.tex_cols));
   }
+  f3 value = f3{D.emissive[0], D.emissive[1], D.emissive[2]};
+  if (J.lighting) {  // :279-304, FAST_LIGHT
+    const f3 position = add3(add3(scale3(k[0], v0.p), scale3(k[1], v1.p)),
+                             scale3(k[2], v2.p));
+    const f3 L = f3{J.light[0], J.light[1], J.light[2]};
+    const f3 lightdir = J.light_is_point ? norm3(sub3(L, position)) : L;
+    const f3 normal = add3(add3(scale3(k[0], v0.n), scale3(k[1], v1.n)),
+                           scale3(k[2], v2.n));
+    const float diff = amax(0.0f, dot3(normal, lightdir)) / len3(normal) + J.ambient;
+    value = add3(value, scale3(diff, color));
+    if (D.shininess != 0.0f) {
+      const f3 view = norm3(position);
+      const f3 hv = sub3(lightdir, view);
+      const float sp = powf(amax(dot3(hv, normal), 0.0f) / (len3(hv) * len3(normal)),
+                            D.shininess);
+      value = add3(value, scale3(sp, f3{D.specular[0], D.specular[1], D.specular[2]}));
+    }
+    if (J.normals && J.save_normals) {
+      J.normals[pix * 3 + 0] = normal.x;
+      J.normals[pix * 3 + 1] = normal.y;
+      J.normals[pix * 3 + 2] = normal.z;
+    }
+  } else if (J.normals && J.save_normals) {
+    J.normals[pix * 3 + 0] = 0.0f;
+    J.normals[pix * 3 + 1] = 0.0f;
+    J.normals[pix * 3 + 2] = 1.0f;
+  }
+  if (J.rgb8) {
+    J.rgb8[pix * 3 + 0] = quantize(value.x);
+    J.rgb8[pix * 3 + 1] = quantize(value.y);
+    J.rgb8[pix * 3 + 2] = quantize(value.z);
+  }
+  if (J.rgb) {
+    J.rgb[pix * 3 + 0] = value.x;
+    J.rgb[pix * 3 + 1] = value.y;
+    J.rgb[pix * 3 + 2] = value.z;
+  }
+  if (J.depth) J.depth[pix] = z;
+  if (J.winner) J.winner[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
 }
 
 // Statistics of the frame: sum the per-workgroup counts of the geometry and
@@ -694,121 +700,24 @@ __device__ void reduce_stats(const BatchArgs& a, int tid) {
   }
 }
 
+// (e): one lane per pixel; a workgroup shades a kShadeW x kShadeH block of
+// one job (each wavefront one kShadeW-pixel row segment).
 __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
-  __shared__ ShadeLds L;
+  __shared__ int64_t dcache[kMaxLdsDraws];
   const int tid = threadIdx.x;
   const int b = (int)xcd_remap(blockIdx.x, a.shade_blocks);
   const int j = search_last(a.job_shade_base, 0, a.njobs - 1, b);
   const JobDesc& J = a.jobs[j];
-  const int ncached = fill_draw_cache(L.dcache, a, J, tid, kShadeBlock);
+  const int ncached = fill_draw_cache(dcache, a, J, tid, kShadeBlock);
+  __syncthreads();
+  if (blockIdx.x == 0) reduce_stats(a, tid);
   const int bxn = (J.W + kShadeW - 1) / kShadeW;
   const int local = b - J.shade_base;
   const int byi = local / bxn;
-  const int xb0 = (local - byi * bxn) * kShadeW, yb0 = byi * kShadeH;
   const int r = tid / kShadeW;
-  const int c = xb0 + (tid - r * kShadeW), row = yb0 + r;
-  const bool inside = c < J.W && row < J.H;
-  const size_t pix = (size_t)row * (size_t)J.W + (size_t)c;
-  for (int i = tid; i < kHashSlots; i += kShadeBlock) L.key[i] = kEmpty;
-  if (tid == 0) L.count = 0;
-  __syncthreads();
-  // Distinct winners of the block -> dense indices.
-  const uint32_t id = inside ? J.win[pix] : kNoWinner;
-  uint32_t slot = 0;
-  if (id != kNoWinner) {
-    slot = (id * 2654435761u) >> 23;  // 9-bit multiplicative hash
-    for (;;) {
-      const uint32_t old = atomicCAS(&L.key[slot], kEmpty, id);
-      if (old == kEmpty) {
-        const uint32_t dense = (uint32_t)atomicAdd(&L.count, 1);
-        L.val[slot] = dense;
-        L.uniq[dense] = id;
-        break;
-      }
-      if (old == id) break;
-      slot = (slot + 1) & (kHashSlots - 1);
-    }
-  }
-  __syncthreads();
-  const int nuniq = L.count;
-  if (blockIdx.x == 0) reduce_stats(a, tid);
-  const bool attrs = J.rgb8 || J.rgb || (J.normals && J.save_normals);
-  for (int u = tid; u < nuniq; u += kShadeBlock) shade_record(a, J, L, ncached, u, xb0, yb0, attrs);
-  __syncthreads();
-  if (!inside) return;
-  if (id == kNoWinner) {
-    store_background(J, pix);
-    return;
-  }
-  const int u = (int)L.val[slot];
-  // Continue the row's recurrence e += n.x (Renderer.cpp:243) to column c.
-  const float n1x = L.n1x[u], n2x = L.n2x[u];
-  float e1 = L.e1[r][u], e2 = L.e2[r][u];
-  for (int x = L.cst[u]; x < c; x++) {
-    e1 = e1 + n1x;
-    e2 = e2 + n2x;
-  }
-  float k[3];
-  const float z = depth_of(e1, e2, L.z0[u], L.z1[u], L.z2[u], J.persp != 0, k);
-  const DrawDesc& D = a.draws[L.draw[u]];
-  const uint32_t wid = id - 2u * (uint32_t)D.tri_begin + D.order_base;  // draw-order index
-  if (!attrs) {
-    if (J.depth) J.depth[pix] = z;
-    if (J.winner) J.winner[pix] = wid;
-    return;
-  }
-  f3 p[3], n[3];
-  f2 uv[3];
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    p[i] = f3{L.attr[3 * i][u], L.attr[3 * i + 1][u], L.attr[3 * i + 2][u]};
-    n[i] = f3{L.attr[9 + 3 * i][u], L.attr[9 + 3 * i + 1][u], L.attr[9 + 3 * i + 2][u]};
-    uv[i] = f2{L.attr[18 + 2 * i][u], L.attr[18 + 2 * i + 1][u]};
-  }
-  f3 color = f3{D.diffuse[0], D.diffuse[1], D.diffuse[2]};
-  if (D.tex) {  // :271-275
-    const float tu = (k[0] * uv[0].x + k[1] * uv[1].x) + k[2] * uv[2].x;
-    const float tv = (k[0] * uv[0].y + k[1] * uv[1].y) + k[2] * uv[2].y;
-    color = texel(D.tex, D.tex_format, D.tex_cols, tex_index(tv, D.tex_rows),
-                  tex_index(tu, D.tex_cols));
-  }
-  f3 value = f3{D.emissive[0], D.emissive[1], D.emissive[2]};
-  if (J.lighting) {  // :279-304, FAST_LIGHT
-    const f3 position = add3(add3(scale3(k[0], p[0]), scale3(k[1], p[1])), scale3(k[2], p[2]));
-    const f3 L3 = f3{J.light[0], J.light[1], J.light[2]};
-    const f3 lightdir = J.light_is_point ? norm3(sub3(L3, position)) : L3;
-    const f3 normal = add3(add3(scale3(k[0], n[0]), scale3(k[1], n[1])), scale3(k[2], n[2]));
-    const float diff = amax(0.0f, dot3(normal, lightdir)) / len3(normal) + J.ambient;
-    value = add3(value, scale3(diff, color));
-    if (D.shininess != 0.0f) {
-      const f3 view = norm3(position);
-      const f3 hv = sub3(lightdir, view);
-      const float sp = powf(amax(dot3(hv, normal), 0.0f) / (len3(hv) * len3(normal)),
-                            D.shininess);
-      value = add3(value, scale3(sp, f3{D.specular[0], D.specular[1], D.specular[2]}));
-    }
-    if (J.normals && J.save_normals) {
-      J.normals[pix * 3 + 0] = normal.x;
-      J.normals[pix * 3 + 1] = normal.y;
-      J.normals[pix * 3 + 2] = normal.z;
-    }
-  } else if (J.normals && J.save_normals) {
-    J.normals[pix * 3 + 0] = 0.0f;
-    J.normals[pix * 3 + 1] = 0.0f;
-    J.normals[pix * 3 + 2] = 1.0f;
-  }
-  if (J.rgb8) {
-    J.rgb8[pix * 3 + 0] = quantize(value.x);
-    J.rgb8[pix * 3 + 1] = quantize(value.y);
-    J.rgb8[pix * 3 + 2] = quantize(value.z);
-  }
-  if (J.rgb) {
-    J.rgb[pix * 3 + 0] = value.x;
-    J.rgb[pix * 3 + 1] = value.y;
-    J.rgb[pix * 3 + 2] = value.z;
-  }
-  if (J.depth) J.depth[pix] = z;
-  if (J.winner) J.winner[pix] = wid;
+  const int c = (local - byi * bxn) * kShadeW + (tid - r * kShadeW), row = byi * kShadeH + r;
+  if (c < J.W && row < J.H)
+    shade_pixel(a, J, dcache, ncached, J.win[(size_t)row * J.W + c], c, row);
 }
 
 // ---- launchers -------------------------------------------------------------
