@@ -140,6 +140,10 @@ typedef struct mr_tuning {
 int mr_ctx_autotune(mr_ctx* ctx, const mr_frame* frames, int32_t nframes,
                     const struct mr_outputs* outs, int32_t reps,
                     const char* cache_path, mr_tuning* best);
+/* Largest number of (tile, triangle) pairs one raster workgroup takes; a
+ * tile with more is split into parts merged through device memory
+ * (64 .. 1<<20, default 1024). */
+int mr_ctx_set_frag_size(mr_ctx* ctx, int32_t pairs);
 /* Bin-list capacity in (tile, triangle) entries per input triangle
  * (>= 1); overflow re-runs the frame with a larger list. */
 int mr_ctx_set_bin_factor(mr_ctx* ctx, float entries_per_triangle);

@@ -114,6 +114,9 @@ class Context:
     def set_raster_threads(self, n: int) -> None:
         check(self.lib.mr_ctx_set_raster_threads(self.h, n), "mr_ctx_set_raster_threads")
 
+    def set_frag_size(self, pairs: int) -> None:
+        check(self.lib.mr_ctx_set_frag_size(self.h, pairs), "mr_ctx_set_frag_size")
+
     def set_bin_factor(self, f: float) -> None:
         check(self.lib.mr_ctx_set_bin_factor(self.h, f), "mr_ctx_set_bin_factor")
 

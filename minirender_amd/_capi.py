@@ -102,6 +102,7 @@ SIGNATURES = {
     "mr_ctx_set_tile": (C.c_int, [_P, _I, _I]),
     "mr_ctx_set_bin_factor": (C.c_int, [_P, _F]),
     "mr_ctx_set_raster_threads": (C.c_int, [_P, _I]),
+    "mr_ctx_set_frag_size": (C.c_int, [_P, _I]),
     "mr_ctx_set_profiling": (C.c_int, [_P, _I]),
     "mr_ctx_set_graphs": (C.c_int, [_P, _I]),
     "mr_ctx_get_stats": (C.c_int, [_P, C.POINTER(mr_stats)]),

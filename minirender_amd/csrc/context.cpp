@@ -599,6 +599,13 @@ int mr_ctx_set_tile(mr_ctx* c, int32_t tw, int32_t th) {
   return MR_OK;
 }
 
+int mr_ctx_set_frag_size(mr_ctx* c, int32_t pairs) {
+  if (!c || pairs < 64 || pairs > (1 << 20)) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  c->frag_size = (uint32_t)pairs;
+  return MR_OK;
+}
+
 int mr_ctx_set_raster_threads(mr_ctx* c, int32_t threads) {
   if (!c) return MR_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
