@@ -271,8 +271,17 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
     bool clipped = false;
     uint32_t job = 0;
     int tbase = 0, tiles_x = 1;
+    // The wave's draw: one scalar search for its first triangle (a uniform
+    // value, so the loads are scalar), then lanes past the next draw's start
+    // step forward (draw boundaries inside a wavefront are rare).
+    int dw = search_last(a.draw_begin, 0, a.ndraws - 1,
+                         (int64_t)__builtin_amdgcn_readfirstlane((int)wfirst));
+    dw = __builtin_amdgcn_readfirstlane(dw);
+    const int64_t next_begin = dw + 1 < a.ndraws ? a.draw_begin[dw + 1] : 0x7FFFFFFFFFFFFFFFll;
     if (g < a.total_tris) {
-      const int d = search_last(a.draw_begin, 0, a.ndraws - 1, g);
+      int d = dw;
+      if (g >= next_begin)
+        while (d + 1 < a.ndraws && a.draw_begin[d + 1] <= g) d++;
       const DrawDesc& D = a.draws[d];
       const JobDesc& J = a.jobs[D.job];
       ns = tri_setup(D, J, (int)(g - D.tri_begin), s0, sub0, s1, sub1, clipped);
@@ -385,16 +394,36 @@ __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
   const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
   const int64_t nw = (int64_t)gridDim.x * 4;
   u64 nitems = 0;
-  for (int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); r < a.nregions; r += nw) {
-    const uint32_t cnt = a.region_count[r];
+  // The next region's count and first 64 items are loaded while the current
+  // region is scattered (both loads are independent: every region slot is
+  // allocated, slots past the count are read but not used).
+  int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  uint32_t cnt = 0;
+  BinItem first = BinItem{0, 0, 0, 0, 0, 0};
+  if (r < a.nregions) {
+    cnt = a.region_count[r];
+    first = a.items[r * kRegion + lane];
+  }
+  while (r < a.nregions) {
+    const int64_t rn = r + nw;
+    uint32_t cnt_n = 0;
+    BinItem first_n = BinItem{0, 0, 0, 0, 0, 0};
+    if (rn < a.nregions) {
+      cnt_n = a.region_count[rn];
+      first_n = a.items[rn * kRegion + lane];
+    }
     nitems += cnt;
     for (uint32_t j = 0; j < cnt; j += 64) {
       const bool act = j + lane < cnt;
-      BinItem it = act ? a.items[r * kRegion + j + lane] : BinItem{0, 0, 0, 0, 0, 0};
-      const JobDesc& J = a.jobs[it.job];
+      BinItem it = first;
+      if (j > 0) it = act ? a.items[r * kRegion + j + lane] : BinItem{0, 0, 0, 0, 0, 0};
+      const JobDesc& J = a.jobs[act ? it.job : 0u];
       wave_tile_pairs<true>(a, act, it.id, J.tile_base, J.tiles_x, it.bx0 >> sw,
                             it.bx1 >> sw, it.by0 >> sh, it.by1 >> sh, a.tile_cursor);
     }
+    r = rn;
+    cnt = cnt_n;
+    first = first_n;
   }
   __shared__ unsigned long long sh_items;
   if (threadIdx.x == 0) sh_items = 0;
