@@ -35,10 +35,10 @@ struct DrawDesc {
   int32_t tex_rows, tex_cols;
   int32_t ntri;
   int32_t job;
-  int32_t pad0;
+  int32_t tile_base;    // the job's first global tile and tiles per row
   int64_t tri_begin;    // first flat triangle index of this draw in the batch
   uint32_t order_base;  // 2 * triangles of earlier draws of the same job
-  uint32_t pad1;
+  int32_t tiles_x;
 };
 
 // One frame of a batch.
@@ -65,11 +65,11 @@ struct JobDesc {
   float* normals;
 };
 
-// A surviving (sub-)triangle as the binner sees it: its draw-order id
-// (2 * flat triangle index + clip sub-index) and clamped pixel bbox.
+// A surviving (sub-)triangle as the binner sees it: its id (2 * flat
+// triangle index + clip sub-index), its draw and clamped pixel bbox.
 struct BinItem {
   uint32_t id;
-  uint32_t job;
+  uint32_t draw;
   uint16_t bx0, bx1, by0, by1;
 };
 static_assert(sizeof(BinItem) == 16, "BinItem must be 16 bytes");

@@ -205,6 +205,8 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
       }
       D.ntri = m.ntri;
       D.job = j;
+      D.tile_base = J.tile_base;
+      D.tiles_x = J.tiles_x;
       D.tri_begin = plan.total_tris;
       D.order_base = (uint32_t)order;
       order += 2ull * (uint64_t)m.ntri;
@@ -245,7 +247,7 @@ static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins
   if ((rc = dev_alloc(c, s.tile_heavy, nt, true))) return rc;
   if ((rc = dev_alloc(c, s.tile_done, nt, true))) return rc;
   if ((rc = dev_alloc(c, s.parts, parts * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.bins, (size_t)bins_cap * 4, true))) return rc;
+  if ((rc = dev_alloc(c, s.bins, (size_t)bins_cap * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.heavy_keys, (size_t)heavy_slots * npt, true))) return rc;
   if ((rc = dev_alloc(c, s.scan_status, (size_t)std::max(scb, 1) * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.counters, kNumCounters * 8, true))) return rc;
@@ -271,8 +273,8 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.tile_done = (uint32_t*)s.tile_done.p;
   a.parts = (uint2*)s.parts.p;
   a.parts_cap = (uint32_t)std::min<size_t>(s.parts.bytes / 8, 0xFFFFFFF0u);
-  a.bins = (uint32_t*)s.bins.p;
-  a.bins_cap = s.bins.bytes / 4;
+  a.bins = (uint2*)s.bins.p;
+  a.bins_cap = s.bins.bytes / 8;
   if (a.bins_cap > 0xFFFFFFF0ull) a.bins_cap = 0xFFFFFFF0ull;
   a.heavy_keys = (unsigned long long*)s.heavy_keys.p;
   a.heavy_cap = (uint32_t)(s.heavy_keys.bytes / ((size_t)c->tile_w * c->tile_h * 8));
@@ -320,7 +322,7 @@ static int launch_stages(const BatchArgs& a, Slot& s, bool profile) {
 static uint64_t bins_wanted(mr_ctx* c, const Slot& s, const BatchPlan& p) {
   uint64_t want = std::max<uint64_t>(
       (uint64_t)((double)p.total_tris * c->bin_factor) + (uint64_t)p.ntiles, 1u << 16);
-  return std::max<uint64_t>(want, s.sc.bins.bytes / 4);
+  return std::max<uint64_t>(want, s.sc.bins.bytes / 8);
 }
 
 int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
@@ -415,7 +417,7 @@ int finish_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
     int rc;
     if (hc[kBinOverflow]) {
       const uint64_t need = hc[kBinEntries] + hc[kBinEntries] / 4 + 1024;
-      if ((rc = dev_alloc(c, x.bins, (size_t)need * 4, true))) return rc;
+      if ((rc = dev_alloc(c, x.bins, (size_t)need * 8, true))) return rc;
     }
     if (hc[kHeavyOverflow]) {
       s.heavy_slots = (uint32_t)(hc[kHeavy] + hc[kHeavy] / 4 + 16);

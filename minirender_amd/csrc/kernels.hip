@@ -192,7 +192,7 @@ __device__ inline void tri_setup_sub(const DrawDesc& D, const JobDesc& J, int t,
 // group; all group leaders issue their atomic in one instruction.
 // kScatter: also write the owner's id at the slot the atomic returned.
 template <bool kScatter>
-__device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id,
+__device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id, uint32_t draw,
                                 int tbase, int tiles_x, int tx0, int tx1,
                                 int ty0, int ty1, uint32_t* ctr) {
   const int lane = threadIdx.x & 63;
@@ -242,11 +242,11 @@ __device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id,
     uint32_t old = 0;
     if (ok && lane == leader) old = atomicAdd(&ctr[(size_t)key * kCtrStride], gsize);
     if (kScatter) {
-      const uint32_t oid = __shfl(id, lo);
+      const uint32_t oid = __shfl(id, lo), odraw = __shfl(draw, lo);
       old = __shfl(old, leader);
       if (ok) {
         const uint32_t slot = old + rank;
-        if ((u64)slot < a.bins_cap) a.bins[slot] = oid;
+        if ((u64)slot < a.bins_cap) a.bins[slot] = make_uint2(oid, odraw);
         else atomicOr(&a.counters[kBinOverflow], 1ull);
       }
     }
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
     s1.bx0 = s1.bx1 = s1.by0 = s1.by1 = 0;
     int sub0 = 0, sub1 = 0, ns = 0;
     bool clipped = false;
-    uint32_t job = 0;
+    uint32_t draw = 0;
     int tbase = 0, tiles_x = 1;
     // The wave's draw: one scalar search for its first triangle (a uniform
     // value, so the loads are scalar), then lanes past the next draw's start
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
       const DrawDesc& D = a.draws[d];
       const JobDesc& J = a.jobs[D.job];
       ns = tri_setup(D, J, (int)(g - D.tri_begin), s0, sub0, s1, sub1, clipped);
-      job = (uint32_t)D.job;
+      draw = (uint32_t)d;
       tbase = J.tile_base;
       tiles_x = J.tiles_x;
     }
@@ -298,15 +298,15 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
     const int64_t pos = region * kRegion + __popcll(b1 & lt) + __popcll(b2 & lt);
     const uint32_t id0 = (uint32_t)(2 * g + sub0), id1 = (uint32_t)(2 * g + sub1);
     if (ns >= 1)
-      a.items[pos] = BinItem{id0, job, (uint16_t)s0.bx0, (uint16_t)s0.bx1,
+      a.items[pos] = BinItem{id0, draw, (uint16_t)s0.bx0, (uint16_t)s0.bx1,
                              (uint16_t)s0.by0, (uint16_t)s0.by1};
     if (ns == 2)
-      a.items[pos + 1] = BinItem{id1, job, (uint16_t)s1.bx0, (uint16_t)s1.bx1,
+      a.items[pos + 1] = BinItem{id1, draw, (uint16_t)s1.bx0, (uint16_t)s1.bx1,
                                  (uint16_t)s1.by0, (uint16_t)s1.by1};
-    wave_tile_pairs<false>(a, ns >= 1, id0, tbase, tiles_x, s0.bx0 >> sw,
+    wave_tile_pairs<false>(a, ns >= 1, id0, draw, tbase, tiles_x, s0.bx0 >> sw,
                            s0.bx1 >> sw, s0.by0 >> sh, s0.by1 >> sh, a.tile_count);
     if (b2)
-      wave_tile_pairs<false>(a, ns == 2, id1, tbase, tiles_x, s1.bx0 >> sw,
+      wave_tile_pairs<false>(a, ns == 2, id1, draw, tbase, tiles_x, s1.bx0 >> sw,
                              s1.bx1 >> sw, s1.by0 >> sh, s1.by1 >> sh, a.tile_count);
   }
   __shared__ uint32_t sh_clipped;
@@ -417,8 +417,8 @@ __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
       const bool act = j + lane < cnt;
       BinItem it = first;
       if (j > 0) it = act ? a.items[r * kRegion + j + lane] : BinItem{0, 0, 0, 0, 0, 0};
-      const JobDesc& J = a.jobs[act ? it.job : 0u];
-      wave_tile_pairs<true>(a, act, it.id, J.tile_base, J.tiles_x, it.bx0 >> sw,
+      const DrawDesc& D = a.draws[act ? it.draw : 0u];
+      wave_tile_pairs<true>(a, act, it.id, it.draw, D.tile_base, D.tiles_x, it.bx0 >> sw,
                             it.bx1 >> sw, it.by0 >> sh, it.by1 >> sh, a.tile_cursor);
     }
     r = rn;
@@ -456,7 +456,6 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
   __shared__ uint32_t setup_id[NT];
   __shared__ int row_start[NT];
   __shared__ int wave_rows[NT / 64];
-  __shared__ int64_t dcache[kMaxLdsDraws];
   __shared__ int sh_last;
   const int tid = threadIdx.x;
   const int64_t nparts_all = (int64_t)min(a.counters[kParts], (u64)a.parts_cap);
@@ -478,7 +477,6 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
     if (b >= nparts_all && nparts > 1) return;  // rendered through its parts
     const int j = search_last(a.job_tile_base, 0, a.njobs - 1, tile);
     const JobDesc& J = a.jobs[j];
-    const int ncached = fill_draw_cache(dcache, a, J, tid, NT);
     const bool persp = J.persp != 0;
     const int local = tile - J.tile_base;
     const int tyi = local / J.tiles_x, txi = local - tyi * J.tiles_x;
@@ -495,9 +493,10 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
       // One lane per entry: setup, and the tile rows it covers.
       int nrows = 0;
       if (base + tid < e_end) {
-        const uint32_t id = a.bins[off + base + tid];
+        const uint2 be = a.bins[off + base + tid];
+        const uint32_t id = be.x;
         const int64_t g = id >> 1;
-        const DrawDesc& D = a.draws[lookup_draw(dcache, ncached, a, J, g)];
+        const DrawDesc& D = a.draws[be.y];
         TriSetup s;
         tri_setup_sub(D, J, (int)(g - D.tri_begin), (int)(id & 1u), s);
         nrows = min(s.by1, y0 + TH - 1) - max(s.by0, y0) + 1;
