@@ -116,12 +116,13 @@ int mr_ctx_get_stream(mr_ctx* ctx, void** stream);
  * (each of 8, 16, 32).  Defaults to 16 x 16. */
 int mr_ctx_set_tile(mr_ctx* ctx, int32_t tile_w, int32_t tile_h);
 /* Workgroup size of the tile raster for the current tile size (0 = the
- * default; 8x8: 64 or 128, 16x16: 128, 256 or 512, 32x32: 256 or 512).
+ * default; 8x8: 64 or 128; 16x16 and 32x32: 64, 128, 256 or 512).
  * mr_ctx_set_tile resets it to the default. */
 int mr_ctx_set_raster_threads(mr_ctx* ctx, int32_t threads);
 /* Autotuner (no counterpart in the reference).  Renders `frames` (n >= 1,
  * with outputs `outs`) under every built combination of tile size (8, 16,
- * 32), raster workgroup size and split size of crowded tiles, `reps` timed
+ * 32), raster workgroup size and split size of crowded tiles (256, 1024,
+ * 4096 pairs), `reps` timed
  * runs each (HIP events on the context's stream), keeps the fastest and
  * sets the bin-list capacity to the measured pairs per triangle + 25%.  The
  * choice is applied to the context and returned in *best.  With cache_path,

@@ -782,8 +782,8 @@ int default_raster_threads(int tw) { return tw == 8 ? 64 : 256; }
 bool raster_threads_supported(int tw, int nt) {
   switch (tw) {
     case 8: return nt == 64 || nt == 128;
-    case 16: return nt == 128 || nt == 256 || nt == 512;
-    case 32: return nt == 256 || nt == 512;
+    case 16:
+    case 32: return nt == 64 || nt == 128 || nt == 256 || nt == 512;
     default: return false;
   }
 }
@@ -799,9 +799,12 @@ hipError_t launch_tile(const BatchArgs& a, hipStream_t s) {
   if (a.tile_w != a.tile_h) return hipErrorInvalidValue;
   MR_TILE_CASE(8, 64)
   MR_TILE_CASE(8, 128)
+  MR_TILE_CASE(16, 64)
   MR_TILE_CASE(16, 128)
   MR_TILE_CASE(16, 256)
   MR_TILE_CASE(16, 512)
+  MR_TILE_CASE(32, 64)
+  MR_TILE_CASE(32, 128)
   MR_TILE_CASE(32, 256)
   MR_TILE_CASE(32, 512)
   return hipErrorInvalidValue;

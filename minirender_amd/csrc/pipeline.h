@@ -72,8 +72,8 @@ hipError_t launch_bin(const BatchArgs& a, hipStream_t s);
 hipError_t launch_tile(const BatchArgs& a, hipStream_t s);
 hipError_t launch_shade(const BatchArgs& a, hipStream_t s);
 bool tile_supported(int tile_w, int tile_h);
-// Raster workgroup sizes built per tile size: 8x8 {64, 128}, 16x16
-// {128, 256, 512}, 32x32 {256, 512}; the default is 64 for 8x8, else 256.
+// Raster workgroup sizes built per tile size: 8x8 {64, 128}, 16x16 and
+// 32x32 {64, 128, 256, 512}; the default is 64 for 8x8, else 256.
 bool raster_threads_supported(int tile_w, int threads);
 int default_raster_threads(int tile_w);
 

@@ -117,7 +117,7 @@ int mr_ctx_autotune(mr_ctx* c, const mr_frame* frames, int32_t n, const mr_outpu
   for (int t : tiles)
     for (int nt : nts)
       if (raster_threads_supported(t, nt))
-        for (int frag : {1024, 4096}) cands.push_back({t, nt, frag});
+        for (int frag : {256, 1024, 4096}) cands.push_back({t, nt, frag});
   int rc = MR_OK;
   double best_ms = 1e300;
   Cand best_c = {16, 256, 1024};

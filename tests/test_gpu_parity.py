@@ -52,8 +52,8 @@ def test_fuzz_scenes(gpu_ctx, seed):
     check(gpu_ctx, fuzz_frame(seed), f"fuzz seed {seed}")
 
 
-@pytest.mark.parametrize("tile,threads", [(8, 64), (8, 128), (16, 128), (16, 256), (16, 512),
-                                          (32, 256), (32, 512)])
+@pytest.mark.parametrize("tile,threads", [(8, 64), (8, 128), (16, 64), (16, 128), (16, 256),
+                                          (16, 512), (32, 64), (32, 128), (32, 256), (32, 512)])
 def test_tile_sizes(gpu_ctx, tile, threads):
     gpu_ctx.set_tile(tile, tile)
     gpu_ctx.set_raster_threads(threads)
