@@ -1,7 +1,7 @@
 // tune.cpp -- autotuner of the launch configuration (C-ABI mr_ctx_autotune).
 //
 // For a representative batch of frames it times every built combination of
-// tile size and raster workgroup size, and two split sizes of crowded tiles,
+// tile size, raster workgroup size and split size of crowded tiles,
 // with HIP events on the context's stream, keeps the fastest, and sizes the
 // bin list from the measured (tile, triangle) pairs per triangle.  Results
 // can be kept in a small text cache keyed by the GPU and the batch's shape
