@@ -552,19 +552,26 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
     } else {
       // Split tile: merge into the tile's global key buffer; the last part
       // to finish writes the winners.
+      // No device-wide fence here: on this GPU each one writes back and
+      // invalidates the XCD's whole L2.  The merges are returning
+      // device-scope atomics whose results are consumed before the barrier,
+      // so they are performed before the part counter is incremented; the
+      // last part reads the merged keys with device-scope atomic loads.
       const uint32_t slot = a.tile_heavy[tile];
       if (slot != kNoSlot) {
         u64* gk = a.heavy_keys + (u64)slot * NP;
+        u64 seen = ~0ull;
         for (int p = tid; p < NP; p += NT)
-          if (keys[p] != ~0ull) atomicMin(&gk[p], keys[p]);
-        __threadfence();
+          if (keys[p] != ~0ull)
+            seen &= __hip_atomic_fetch_min(&gk[p], keys[p], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+        if (seen == 0ull) sh_last = 0;  // consumes the returned values
         __syncthreads();
         if (tid == 0)
-          sh_last = __hip_atomic_fetch_add(&a.tile_done[tile], 1u, __ATOMIC_ACQ_REL,
+          sh_last = __hip_atomic_fetch_add(&a.tile_done[tile], 1u, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT) == nparts - 1;
         __syncthreads();
         if (sh_last) {
-          __threadfence();
           for (int p = tid; p < NP; p += NT) {
             const int c = x0 + (p % TW), row = y0 + (p / TW);
             const u64 k = __hip_atomic_load(&gk[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
