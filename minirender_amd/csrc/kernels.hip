@@ -413,7 +413,47 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
     sh_excl = excl;
     if (blk == a.scan_blocks - 1) a.counters[kBinEntries] = excl + btot;
   }
+  // Split (heavy) tiles: the workgroup reserves its merge slots and part
+  // list entries with one atomic each (same-address atomics serialise).
+  uint32_t nh = 0, npt = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++)
+    if (base + i < a.ntiles && v[i] > a.frag_size) {
+      nh++;
+      npt += (v[i] + a.frag_size - 1) / a.frag_size;
+    }
+  uint32_t hx = nh, px = npt;  // wave inclusive scans
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t yh = __shfl_up(hx, o), yp = __shfl_up(px, o);
+    if (lane >= o) {
+      hx += yh;
+      px += yp;
+    }
+  }
+  __shared__ uint32_t sh_h[kScanBlock / 64], sh_p[kScanBlock / 64];
+  __shared__ u64 sh_hbase, sh_pbase;
+  if (lane == 63) {
+    sh_h[wave] = hx;
+    sh_p[wave] = px;
+  }
   __syncthreads();
+  uint32_t hpre = 0, ppre = 0, htot = 0, ptot = 0;
+#pragma unroll
+  for (int w = 0; w < kScanBlock / 64; w++) {
+    if (w < wave) {
+      hpre += sh_h[w];
+      ppre += sh_p[w];
+    }
+    htot += sh_h[w];
+    ptot += sh_p[w];
+  }
+  if (tid == 0) {
+    sh_hbase = htot ? atomicAdd(&a.counters[kHeavy], (u64)htot) : 0;
+    sh_pbase = ptot ? atomicAdd(&a.counters[kParts], (u64)ptot) : 0;
+  }
+  __syncthreads();
+  u64 hs = sh_hbase + hpre + (hx - nh), ps = sh_pbase + ppre + (px - npt);
   u64 run = sh_excl + wpre + (x - sum);
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
@@ -425,14 +465,14 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
       uint32_t slot = kNoSlot;
       if (v[i] > a.frag_size) {
         const uint32_t nparts = (v[i] + a.frag_size - 1) / a.frag_size;
-        const u64 s = atomicAdd(&a.counters[kHeavy], 1ull);
-        const u64 e = atomicAdd(&a.counters[kParts], (u64)nparts);
-        if (s < a.heavy_cap && e + nparts <= a.parts_cap) {
-          slot = (uint32_t)s;
-          for (uint32_t p = 0; p < nparts; p++) a.parts[e + p] = make_uint2((uint32_t)t, p);
+        if (hs < a.heavy_cap && ps + nparts <= a.parts_cap) {
+          slot = (uint32_t)hs;
+          for (uint32_t p = 0; p < nparts; p++) a.parts[ps + p] = make_uint2((uint32_t)t, p);
         } else {
           atomicOr(&a.counters[kHeavyOverflow], 1ull);
         }
+        hs++;
+        ps += nparts;
       }
       a.tile_heavy[t] = slot;
     }
@@ -553,7 +593,10 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
   __shared__ int wave_rows[NT / 64];
   __shared__ int sh_last;
   const int tid = threadIdx.x;
-  const int64_t nparts_all = (int64_t)min(a.counters[kParts], (u64)a.parts_cap);
+  // After a merge-slot or part-list overflow the part list is incomplete:
+  // split tiles are then skipped (the frame is re-run with larger buffers).
+  const int64_t nparts_all =
+      a.counters[kHeavyOverflow] ? 0 : (int64_t)min(a.counters[kParts], (u64)a.parts_cap);
   const int64_t b = blockIdx.x;
   int tile, part;
   if (b < nparts_all) {
