@@ -118,6 +118,19 @@ __device__ inline int fill_draw_cache(int64_t* dcache, const BatchArgs& a,
   return n;
 }
 
+// A draw descriptor read through the constant address space: for a
+// wavefront-uniform index its words are fetched with scalar loads (into
+// SGPRs) instead of one vector load per field.
+__device__ inline DrawDesc uniform_draw(const BatchArgs& a, int d) {
+  typedef const __attribute__((address_space(4))) uint32_t* ConstWords;
+  const ConstWords src = (ConstWords)(a.draws + d);
+  DrawDesc out;
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(DrawDesc) / 4); i++) dst[i] = src[i];
+  return out;
+}
+
 __device__ inline f3 load3(const float* p, int i) {
   return f3{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
 }
@@ -331,12 +344,15 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
       int d = dw;
       if (g >= next_begin)
         while (d + 1 < a.ndraws && a.draw_begin[d + 1] <= g) d++;
-      const DrawDesc& D = a.draws[d];
-      const JobDesc& J = a.jobs[D.job];
-      ns = tri_setup(D, J, (int)(g - D.tri_begin), s0, sub0, s1, sub1, clipped);
+      auto setup = [&](const DrawDesc& D) {
+        const JobDesc& J = a.jobs[D.job];
+        ns = tri_setup(D, J, (int)(g - D.tri_begin), s0, sub0, s1, sub1, clipped);
+        tbase = D.tile_base;
+        tiles_x = D.tiles_x;
+      };
+      if (__ballot(d == dw) == __ballot(1)) setup(uniform_draw(a, dw));
+      else setup(a.draws[d]);
       draw = (uint32_t)d;
-      tbase = J.tile_base;
-      tiles_x = J.tiles_x;
     }
     nclipped += (uint32_t)__popcll(__ballot(clipped));
     const u64 b1 = __ballot(ns >= 1), b2 = __ballot(ns == 2);
@@ -634,9 +650,15 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
         const uint2 be = a.bins[off + base + tid];
         const uint32_t id = be.x;
         const int64_t g = id >> 1;
-        const DrawDesc& D = a.draws[be.y];
         TriSetup s;
-        tri_setup_sub(D, J, (int)(g - D.tri_begin), (int)(id & 1u), s);
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(be.y);
+        if (__ballot(be.y == d0) == __ballot(1)) {
+          const DrawDesc Du = uniform_draw(a, (int)d0);
+          tri_setup_sub(Du, J, (int)(g - Du.tri_begin), (int)(id & 1u), s);
+        } else {
+          const DrawDesc& D = a.draws[be.y];
+          tri_setup_sub(D, J, (int)(g - D.tri_begin), (int)(id & 1u), s);
+        }
         nrows = min(s.by1, y0 + TH - 1) - max(s.by0, y0) + 1;
         setups[tid] = s;
         setup_id[tid] = id;
@@ -722,34 +744,11 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
   }
 }
 
-// Deferred shading of one pixel from its winner (Renderer.cpp:271-305).
-__device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
-                            const int64_t* dcache, int ncached, uint32_t id,
-                            int c, int row) {
-  const size_t pix = (size_t)row * (size_t)J.W + (size_t)c;
-  if (id == kNoWinner) {  // clear(), Renderer.cpp:113-119
-    if (J.rgb8) {
-      J.rgb8[pix * 3 + 0] = quantize(J.bg[0]);
-      J.rgb8[pix * 3 + 1] = quantize(J.bg[1]);
-      J.rgb8[pix * 3 + 2] = quantize(J.bg[2]);
-    }
-    if (J.depth) J.depth[pix] = kClearDepth;
-    if (J.winner) J.winner[pix] = kNoWinner;
-    if (J.rgb) {
-      J.rgb[pix * 3 + 0] = J.bg[0];
-      J.rgb[pix * 3 + 1] = J.bg[1];
-      J.rgb[pix * 3 + 2] = J.bg[2];
-    }
-    if (J.normals && J.save_normals) {
-      J.normals[pix * 3 + 0] = 0.0f;
-      J.normals[pix * 3 + 1] = 0.0f;
-      J.normals[pix * 3 + 2] = 1.0f;
-    }
-    return;
-  }
+// Shading of a covered pixel whose winner `id` belongs to draw D.
+__device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, const DrawDesc& D,
+                                          uint32_t id, int c, int row, size_t pix) {
   const int64_t g = id >> 1;
   const int sub = (int)(id & 1u);
-  const DrawDesc& D = a.draws[lookup_draw(dcache, ncached, a, J, g)];
   const int tri = (int)(g - D.tri_begin);
   TriSetup s;
   tri_setup_sub(D, J, tri, sub, s);
@@ -841,6 +840,42 @@ __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
   }
   if (J.depth) J.depth[pix] = z;
   if (J.winner) J.winner[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
+}
+
+// Deferred shading of one pixel from its winner (Renderer.cpp:271-305).
+__device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
+                            const int64_t* dcache, int ncached, uint32_t id,
+                            int c, int row) {
+  const size_t pix = (size_t)row * (size_t)J.W + (size_t)c;
+  if (id == kNoWinner) {  // clear(), Renderer.cpp:113-119
+    if (J.rgb8) {
+      J.rgb8[pix * 3 + 0] = quantize(J.bg[0]);
+      J.rgb8[pix * 3 + 1] = quantize(J.bg[1]);
+      J.rgb8[pix * 3 + 2] = quantize(J.bg[2]);
+    }
+    if (J.depth) J.depth[pix] = kClearDepth;
+    if (J.winner) J.winner[pix] = kNoWinner;
+    if (J.rgb) {
+      J.rgb[pix * 3 + 0] = J.bg[0];
+      J.rgb[pix * 3 + 1] = J.bg[1];
+      J.rgb[pix * 3 + 2] = J.bg[2];
+    }
+    if (J.normals && J.save_normals) {
+      J.normals[pix * 3 + 0] = 0.0f;
+      J.normals[pix * 3 + 1] = 0.0f;
+      J.normals[pix * 3 + 2] = 1.0f;
+    }
+    return;
+  }
+  const int d = lookup_draw(dcache, ncached, a, J, (int64_t)(id >> 1));
+  const int d0 = __builtin_amdgcn_readfirstlane(d);
+  if (__ballot(d == d0) == __ballot(1)) {
+    // One draw for the whole wavefront: its descriptor through scalar loads.
+    const DrawDesc Du = uniform_draw(a, d0);
+    shade_lit(a, J, Du, id, c, row, pix);
+  } else {
+    shade_lit(a, J, a.draws[d], id, c, row, pix);
+  }
 }
 
 // Statistics of the frame: sum the per-workgroup counts of the geometry and
