@@ -181,6 +181,22 @@ __device__ inline int tri_setup(const DrawDesc& D, const JobDesc& J, int t,
   return 0;
 }
 
+// Setup of sub-triangle `sub` of a binned triangle from its eye-space corners.
+__device__ inline void tri_setup_eye(const JobDesc& J, const f3& e0, const f3& e1, const f3& e2,
+                                     int sub, TriSetup& s) {
+  const float zn = J.znear;
+  const bool persp = J.persp != 0;
+  const float w = (float)J.W, h = (float)J.H;
+  if (e0.z > zn || e1.z > zn || e2.z > zn) {
+    f3 a0, a1, a2, c0, c1, c2;
+    clip_positions(zn, e0, e1, e2, a0, a1, a2, c0, c1, c2);
+    if (sub) setup_triangle(J.P, persp, w, h, c0, c1, c2, s);
+    else setup_triangle(J.P, persp, w, h, a0, a1, a2, s);
+    return;
+  }
+  setup_triangle(J.P, persp, w, h, e0, e1, e2, s);
+}
+
 // Setup of sub-triangle `sub` of a binned triangle (which survived setup).
 __device__ inline void tri_setup_sub(const DrawDesc& D, const JobDesc& J, int t,
                                      int sub, TriSetup& s) {
@@ -750,8 +766,34 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   const int64_t g = id >> 1;
   const int sub = (int)(id & 1u);
   const int tri = (int)(g - D.tri_begin);
+  // All index loads first, then all attribute loads: two dependent levels
+  // of memory access instead of four.
+  const int* ip = D.idx + 3 * tri;
+  const int* in = D.nidx + 3 * tri;
+  const int i0 = ip[0], i1 = ip[1], i2 = ip[2];
+  const int n0 = in[0], n1 = in[1], n2 = in[2];
+  int t0 = 0, t1 = 0, t2 = 0;
+  if (D.tidx) {
+    const int* it = D.tidx + 3 * tri;
+    t0 = it[0];
+    t1 = it[1];
+    t2 = it[2];
+  }
+  const f3 P0 = load3(D.pos, i0), P1 = load3(D.pos, i1), P2 = load3(D.pos, i2);
+  const f3 N0 = load3(D.nrm, n0), N1 = load3(D.nrm, n1), N2 = load3(D.nrm, n2);
+  f2 U0 = f2{0.0f, 0.0f}, U1 = U0, U2 = U0;  // Renderer.cpp:371-379
+  if (D.tidx) {
+    U0 = f2{D.uv[2 * t0], D.uv[2 * t0 + 1]};
+    U1 = f2{D.uv[2 * t1], D.uv[2 * t1 + 1]};
+    U2 = f2{D.uv[2 * t2], D.uv[2 * t2 + 1]};
+  }
+  // Eye-space vertices as paintMesh computes them (Renderer.cpp:344-349).
+  Vtx v0, v1, v2;
+  v0.p = xform(D.mv, P0);
+  v1.p = xform(D.mv, P1);
+  v2.p = xform(D.mv, P2);
   TriSetup s;
-  tri_setup_sub(D, J, tri, sub, s);
+  tri_setup_eye(J, v0.p, v1.p, v2.p, sub, s);
   float e1, e2;
   edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
              (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
@@ -766,27 +808,14 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
     return;
   }
 
-  // The triangle's eye-space vertices, recomputed as paintMesh does
-  // (Renderer.cpp:344-379), and re-clipped for a clipped sub-triangle.
-  Vtx v0, v1, v2;
-  {
-    const int* ip = D.idx + 3 * tri;
-    const int* in = D.nidx + 3 * tri;
-    v0.p = xform(D.mv, load3(D.pos, ip[0]));
-    v1.p = xform(D.mv, load3(D.pos, ip[1]));
-    v2.p = xform(D.mv, load3(D.pos, ip[2]));
-    v0.n = xform(D.nm, load3(D.nrm, in[0]));
-    v1.n = xform(D.nm, load3(D.nrm, in[1]));
-    v2.n = xform(D.nm, load3(D.nrm, in[2]));
-    if (D.tidx) {
-      const int* it = D.tidx + 3 * tri;
-      v0.uv = f2{D.uv[2 * it[0]], D.uv[2 * it[0] + 1]};
-      v1.uv = f2{D.uv[2 * it[1]], D.uv[2 * it[1] + 1]};
-      v2.uv = f2{D.uv[2 * it[2]], D.uv[2 * it[2] + 1]};
-    } else {
-      v0.uv = v1.uv = v2.uv = f2{0.0f, 0.0f};
-    }
-  }
+  // Normals and uvs of the triangle, re-clipped for a clipped sub-triangle
+  // (Renderer.cpp:131-161).
+  v0.n = xform(D.nm, N0);
+  v1.n = xform(D.nm, N1);
+  v2.n = xform(D.nm, N2);
+  v0.uv = U0;
+  v1.uv = U1;
+  v2.uv = U2;
   if (v0.p.z > J.znear || v1.p.z > J.znear || v2.p.z > J.znear) {
     Vtx a0, a1, a2, b0, b1, b2;
     clip_triangle(J.znear, v0, v1, v2, a0, a1, a2, b0, b1, b2);
