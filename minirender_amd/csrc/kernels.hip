@@ -282,61 +282,12 @@ __device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id, ui
   }
 }
 
-// Workgroup-local aggregation of per-tile atomics: an LDS hash table from
-// tile to a count (and, when scattering, to the tile's global base and a
-// local cursor).  Neighbouring triangles of a workgroup share few tiles, so
-// one global atomic per distinct tile replaces one per wavefront group.
-constexpr int kAggSlots = 1024;
-constexpr int kAggProbes = 32;
-constexpr uint32_t kAggEmpty = 0xFFFFFFFFu;
-
-struct AggLds {
-  uint32_t key[kAggSlots];
-  uint32_t cnt[kAggSlots];
-  uint32_t base[kAggSlots];
-  uint32_t cur[kAggSlots];
-  int overflow;
-};
-
-__device__ inline uint32_t agg_hash(uint32_t key) { return (key * 2654435761u) >> 22; }
-
-// Slot of `key`, inserting it if absent; -1 when no slot is free within
-// kAggProbes probes.
-__device__ inline int agg_insert(AggLds& L, uint32_t key) {
-  uint32_t h = agg_hash(key);
-  for (int i = 0; i < kAggProbes; i++) {
-    const uint32_t old = atomicCAS(&L.key[h], kAggEmpty, key);
-    if (old == kAggEmpty || old == key) return (int)h;
-    h = (h + 1) & (kAggSlots - 1);
-  }
-  return -1;
-}
-
-// Slot of a key known to be present.
-__device__ inline int agg_find(const AggLds& L, uint32_t key) {
-  uint32_t h = agg_hash(key);
-  while (L.key[h] != key) h = (h + 1) & (kAggSlots - 1);
-  return (int)h;
-}
-
-__device__ inline void agg_clear(AggLds& L, int tid, int nt) {
-  for (int i = tid; i < kAggSlots; i += nt) {
-    L.key[i] = kAggEmpty;
-    L.cnt[i] = 0;
-    L.cur[i] = 0;
-  }
-  if (tid == 0) L.overflow = 0;
-}
-
 // (a)+(b): one lane per input triangle; wavefront w writes its survivors to
 // items[w * kRegion ...] and their number to region_count[w].
 __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
-  __shared__ AggLds agg;
   const int lane = threadIdx.x & 63;
   const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
   uint32_t nclipped = 0;
-  agg_clear(agg, threadIdx.x, 256);
-  __syncthreads();
   for (int64_t base = (int64_t)blockIdx.x * 256; base < a.total_tris;
        base += (int64_t)gridDim.x * 256) {
     const int64_t wfirst = base + (threadIdx.x & ~63);
@@ -385,9 +336,7 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
       a.items[pos + 1] = BinItem{id1, draw, (uint16_t)s1.bx0, (uint16_t)s1.bx1,
                                  (uint16_t)s1.by0, (uint16_t)s1.by1};
     auto count = [&](int key, uint32_t n) -> uint32_t {
-      const int slot = agg_insert(agg, (uint32_t)key);
-      if (slot >= 0) atomicAdd(&agg.cnt[slot], n);
-      else atomicAdd(&a.tile_count[(size_t)key * kCtrStride], n);
+      atomicAdd(&a.tile_count[(size_t)key * kCtrStride], n);
       return 0u;
     };
     wave_tile_pairs<false>(a, ns >= 1, id0, draw, tbase, tiles_x, s0.bx0 >> sw,
@@ -400,9 +349,6 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
   if (threadIdx.x == 0) sh_clipped = 0;
   __syncthreads();
   if (lane == 0 && nclipped) atomicAdd(&sh_clipped, nclipped);
-  // one global add per distinct tile of the workgroup
-  for (int i = threadIdx.x; i < kAggSlots; i += 256)
-    if (agg.key[i] != kAggEmpty) atomicAdd(&a.tile_count[(size_t)agg.key[i] * kCtrStride], agg.cnt[i]);
   __syncthreads();
   if (threadIdx.x == 0 && blockIdx.x < kStatSlots) a.block_stats[blockIdx.x] = sh_clipped;
 }
@@ -515,7 +461,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
 // (c) scatter triangle ids into the bin lists of the tiles they overlap;
 // also resets the merge buffers of split tiles.
 __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
-  __shared__ AggLds agg;
   const int lane = threadIdx.x & 63;
   const u64 np = (u64)a.tile_w * a.tile_h;
   const u64 nheavy = min(a.counters[kHeavy], (u64)a.heavy_cap);
@@ -525,14 +470,12 @@ __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
   const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
   const int64_t nw = (int64_t)gridDim.x * 4;
   u64 nitems = 0;
-  agg_clear(agg, threadIdx.x, 256);
-  // Per step the workgroup's four wavefronts take four consecutive regions
-  // (neighbouring triangles).  (A) count their pairs per tile in LDS, (B) one
-  // global atomic per distinct tile reserves the bin slots, (C) enumerate
-  // again and write each pair at base + a local LDS cursor.  If a tile does
-  // not fit the table, the step falls back to one global atomic per group.
-  // The next step's region count and first items are loaded meanwhile (all
-  // region slots are allocated; slots past the count are not used).
+  auto reserve = [&](int key, uint32_t n) -> uint32_t {
+    return atomicAdd(&a.tile_cursor[(size_t)key * kCtrStride], n);
+  };
+  // The next region's count and first 64 items are loaded while the current
+  // region is scattered (both loads are independent: every region slot is
+  // allocated, slots past the count are read but not used).
   int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   uint32_t cnt = 0;
   BinItem first = BinItem{0, 0, 0, 0, 0, 0};
@@ -540,8 +483,7 @@ __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
     cnt = a.region_count[r];
     first = a.items[r * kRegion + lane];
   }
-  __syncthreads();
-  for (int64_t r0 = (int64_t)blockIdx.x * 4; r0 < a.nregions; r0 += nw) {
+  while (r < a.nregions) {
     const int64_t rn = r + nw;
     uint32_t cnt_n = 0;
     BinItem first_n = BinItem{0, 0, 0, 0, 0, 0};
@@ -549,35 +491,8 @@ __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
       cnt_n = a.region_count[rn];
       first_n = a.items[rn * kRegion + lane];
     }
-    if (r >= a.nregions) cnt = 0;
     nitems += cnt;
-    auto count = [&](int key, uint32_t n) -> uint32_t {
-      const int slot = agg_insert(agg, (uint32_t)key);
-      if (slot >= 0) atomicAdd(&agg.cnt[slot], n);
-      else agg.overflow = 1;
-      return 0u;
-    };
-    for (uint32_t j = 0; j < cnt; j += 64) {  // (A)
-      const bool act = j + lane < cnt;
-      BinItem it = first;
-      if (j > 0) it = act ? a.items[r * kRegion + j + lane] : BinItem{0, 0, 0, 0, 0, 0};
-      const DrawDesc& D = a.draws[act ? it.draw : 0u];
-      wave_tile_pairs<false>(a, act, it.id, it.draw, D.tile_base, D.tiles_x, it.bx0 >> sw,
-                             it.bx1 >> sw, it.by0 >> sh, it.by1 >> sh, count);
-    }
-    __syncthreads();
-    const bool ovf = agg.overflow != 0;
-    if (!ovf)  // (B)
-      for (int i = threadIdx.x; i < kAggSlots; i += 256)
-        if (agg.key[i] != kAggEmpty)
-          agg.base[i] = atomicAdd(&a.tile_cursor[(size_t)agg.key[i] * kCtrStride], agg.cnt[i]);
-    __syncthreads();
-    auto reserve = [&](int key, uint32_t n) -> uint32_t {
-      if (ovf) return atomicAdd(&a.tile_cursor[(size_t)key * kCtrStride], n);
-      const int slot = agg_find(agg, (uint32_t)key);
-      return agg.base[slot] + atomicAdd(&agg.cur[slot], n);
-    };
-    for (uint32_t j = 0; j < cnt; j += 64) {  // (C)
+    for (uint32_t j = 0; j < cnt; j += 64) {
       const bool act = j + lane < cnt;
       BinItem it = first;
       if (j > 0) it = act ? a.items[r * kRegion + j + lane] : BinItem{0, 0, 0, 0, 0, 0};
@@ -585,9 +500,6 @@ __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
       wave_tile_pairs<true>(a, act, it.id, it.draw, D.tile_base, D.tiles_x, it.bx0 >> sw,
                             it.bx1 >> sw, it.by0 >> sh, it.by1 >> sh, reserve);
     }
-    __syncthreads();
-    agg_clear(agg, threadIdx.x, 256);
-    __syncthreads();
     r = rn;
     cnt = cnt_n;
     first = first_n;
