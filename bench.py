@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--workload", default="terrain",
                    choices=["terrain", "megatri", "sphere", "textured"])
     p.add_argument("--tile", type=int, default=0, help="0 = autotune per workload")
+    p.add_argument("--tune-cache", default=None,
+                   help="autotuner cache file: reuse (or record) the launch configuration")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-serving", action="store_true")
@@ -175,7 +177,7 @@ def plain_section(args, torch, mra, frame, wl_name):
         ctx.set_tile(args.tile, args.tile)
         tune = {"tile": args.tile, "fixed": True}
     else:
-        tune = ctx.autotune([frame], outs=[outs], reps=3)
+        tune = ctx.autotune([frame], outs=[outs], reps=3, cache_path=args.tune_cache)
     tile = tune["tile"]
     log(f"plain: {tune}")
     for _ in range(args.warmup):
@@ -303,7 +305,7 @@ def batched_section(args, torch, mra):
             if rc:
                 raise RuntimeError(f"mr_render failed: {rc}")
 
-    tune = ctx.autotune(jobs, outs=outs, reps=2)
+    tune = ctx.autotune(jobs, outs=outs, reps=2, cache_path=args.tune_cache)
     log(f"batched: {tune}")
     reps = max(3, args.steps // 5)
     batch()
