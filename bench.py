@@ -60,6 +60,8 @@ def parse():
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-serving", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run 5 device copies of 256 MiB (to scale HBM byte counters)")
     p.add_argument("--serve-requests", type=int, default=20000)
     p.add_argument("--serve-rate", type=float, default=2000.0)
     p.add_argument("--serve-batch", type=int, default=64)
@@ -420,6 +422,16 @@ def main():
                                    "is described but not built, so --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        n = 256 << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "copies": 5}), flush=True)
+        return
     frame, wl = make_frame(args.workload)
     log(f"workload {args.workload}: {frame.ntri} triangles")
     cpu = None
