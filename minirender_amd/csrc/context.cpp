@@ -301,10 +301,10 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   return a;
 }
 
-static int launch_stages(const BatchArgs& a, Slot& s, bool profile) {
+static int launch_stages(const BatchArgs& a, Slot& s, bool profile, bool clear) {
   hipStream_t st = s.stream;
   if (profile) HIP_TRY(hipEventRecord(s.ev[0], st));
-  HIP_TRY(launch_clear(a, st));
+  if (clear) HIP_TRY(launch_clear(a, st));
   if (profile) HIP_TRY(hipEventRecord(s.ev[1], st));
   HIP_TRY(launch_geometry(a, st));
   if (profile) HIP_TRY(hipEventRecord(s.ev[2], st));
@@ -349,10 +349,23 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   const BatchArgs a = make_args(c, s, p);
   s.last_geom_grid = p.total_tris ? geom_grid(a) : 0;
   s.last_bin_grid = bin_grid(a);
+  // The clear kernel runs only when the counters it zeroes are not known to
+  // be zero: new buffers, more tiles than cleared before, or no geometry
+  // kernel (which clears the batch counters).
+  const bool same_bufs = s.clean_ref[0] == x.tile_count.p && s.clean_ref[1] == x.tile_done.p &&
+                         s.clean_ref[2] == x.scan_status.p;
+  const bool clear = !same_bufs || p.total_tris == 0 || p.ntiles > s.clean_tiles ||
+                     a.scan_blocks > s.clean_scan;
+  // Zero prefixes after this batch: the clear kernel (if run) zeroes this
+  // batch's range, the shading kernel re-zeroes it, entries past it keep
+  // their state.
+  const int64_t next_tiles = std::max<int64_t>(p.ntiles, same_bufs ? s.clean_tiles : 0);
+  const int64_t next_scan = std::max<int64_t>(a.scan_blocks, same_bufs ? s.clean_scan : 0);
+  s.clean_tiles = s.clean_scan = 0;  // until the launches below succeed
   auto issue = [&]() -> int {
     HIP_TRY(hipMemcpyAsync(x.tables.p, x.h_tables.p, p.layout.bytes, hipMemcpyHostToDevice,
                            s.stream));
-    int r = launch_stages(a, s, profile);
+    int r = launch_stages(a, s, profile, clear);
     if (r) return r;
     HIP_TRY(hipMemcpyAsync(x.h_counters.p, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost,
                            s.stream));
@@ -361,7 +374,8 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   // Graphs only on the slot's own stream (a caller's stream may be the null
   // stream, which cannot be captured) and without per-stage events.
   if (c->graphs && !profile && s.own_stream && s.stream == s.own_stream) {
-    const void* hp[3] = {x.h_tables.p, x.h_counters.p, (const void*)p.layout.bytes};
+    const void* hp[4] = {x.h_tables.p, x.h_counters.p, (const void*)p.layout.bytes,
+                         (const void*)(uintptr_t)clear};
     std::vector<unsigned char> key(sizeof(BatchArgs) + sizeof(hp));
     memcpy(key.data(), &a, sizeof(BatchArgs));
     memcpy(key.data() + sizeof(BatchArgs), hp, sizeof(hp));
@@ -401,6 +415,11 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
     if ((rc = issue())) return rc;
   }
   HIP_TRY(hipEventRecord(s.done, s.stream));
+  s.clean_ref[0] = x.tile_count.p;
+  s.clean_ref[1] = x.tile_done.p;
+  s.clean_ref[2] = x.scan_status.p;
+  s.clean_tiles = next_tiles;
+  s.clean_scan = next_scan;
   return MR_OK;
 }
 

@@ -87,6 +87,10 @@ struct Slot {
   };
   std::vector<Graph> graphs;
   int64_t graph_replays = 0, graph_captures = 0;
+  // Leading tile counters / look-back words known to be zero (the shading
+  // kernel re-zeroes what a batch used), valid for the buffers in clean_ref.
+  int64_t clean_tiles = 0, clean_scan = 0;
+  const void* clean_ref[3] = {nullptr, nullptr, nullptr};
 };
 
 }  // namespace mr

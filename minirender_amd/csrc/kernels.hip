@@ -1,7 +1,8 @@
 // kernels.hip -- the rasterization pipeline of minirender's Renderer::render()
 // (src/Renderer.cpp:311-381) as HIP kernels for gfx950 (wave64).
 //
-//   k_clear   counters, look-back words, tile counts and merge counters
+//   k_clear   look-back words, tile counts and merge counters, when the
+//             scratch is new (afterwards the shading kernel leaves them zero)
 //   k_geom    (a)+(b) vertex transform, near clip, projection, setup and
 //             cull of every input triangle (Renderer.cpp:340-380, 163-236).
 //             Each wavefront compacts its survivors into its own region of
@@ -285,6 +286,8 @@ __device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id, ui
 // (a)+(b): one lane per input triangle; wavefront w writes its survivors to
 // items[w * kRegion ...] and their number to region_count[w].
 __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
+  // The batch counters are first used by the scan: clear them here.
+  if (blockIdx.x == 0 && threadIdx.x < kNumCounters) a.counters[threadIdx.x] = 0;
   const int lane = threadIdx.x & 63;
   const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
   uint32_t nclipped = 0;
@@ -858,6 +861,16 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
   const int j = search_last(a.job_shade_base, 0, a.njobs - 1, b);
   const JobDesc& J = a.jobs[j];
   const int ncached = fill_draw_cache(dcache, a, J, tid, kShadeBlock);
+  // The raster and scan are done with the tile counters, part counters and
+  // look-back words: leave them zero for the next batch on this scratch.
+  for (int64_t i = (int64_t)blockIdx.x * kShadeBlock + tid; i < a.ntiles || i < a.scan_blocks;
+       i += (int64_t)a.shade_blocks * kShadeBlock) {
+    if (i < a.ntiles) {
+      a.tile_count[(size_t)i * kCtrStride] = 0;
+      a.tile_done[i] = 0;
+    }
+    if (i < a.scan_blocks) a.scan_status[i] = 0;
+  }
   __syncthreads();
   if (blockIdx.x == 0) reduce_stats(a, tid);
   const int bxn = (J.W + kShadeW - 1) / kShadeW;
