@@ -287,16 +287,29 @@ struct TriSetup {
   int bx0, bx1, by0, by1;    // clamped pixel bbox, inclusive
 };
 
-// Triangle setup of paintTriangle (Renderer.cpp:179-236) for eye-space
-// positions.  Returns false when the triangle is rejected off-screen or
-// back-face culled.
-MR_HD inline bool setup_triangle(const float* P, bool persp, float w, float h,
-                                 f3 a, f3 b, f3 c, TriSetup& r) {
-  f3 ndc0 = htransform(P, a), ndc1 = htransform(P, b), ndc2 = htransform(P, c);
-  const float hw = w / 2.0f, hh = h / 2.0f;
-  float p0x = (1.0f + ndc0.x) * hw, p0y = (1.0f - ndc0.y) * hh;
-  float p1x = (1.0f + ndc1.x) * hw, p1y = (1.0f - ndc1.y) * hh;
-  float p2x = (1.0f + ndc2.x) * hw, p2y = (1.0f - ndc2.y) * hh;
+// A vertex projected to the screen (Renderer.cpp:191-203, 225-227): pixel
+// position and the depth attribute the raster interpolates, -1/z_eye under a
+// perspective projection or NDC z under an orthographic one.  It depends on
+// the vertex alone, so it can be computed once per vertex.
+struct ScreenVtx {
+  float px, py, zv;
+};
+
+MR_HD inline ScreenVtx project_vertex(const float* P, bool persp, float w, float h, f3 e) {
+  const f3 ndc = htransform(P, e);
+  ScreenVtx v;
+  v.px = (1.0f + ndc.x) * (w / 2.0f);
+  v.py = (1.0f - ndc.y) * (h / 2.0f);
+  v.zv = persp ? -1.0f / e.z : ndc.z;
+  return v;
+}
+
+// Triangle setup of paintTriangle (Renderer.cpp:193-236) from projected
+// vertices.  Returns false when the triangle is rejected off-screen or
+// culled (front faces are counter-clockwise in NDC, :207).
+MR_HD inline bool setup_screen(float w, float h, const ScreenVtx& v0, const ScreenVtx& v1,
+                               const ScreenVtx& v2, TriSetup& r) {
+  const float p0x = v0.px, p0y = v0.py, p1x = v1.px, p1y = v1.py, p2x = v2.px, p2y = v2.py;
   float minx = amin(amin(amin(1e30f, p0x), p1x), p2x);
   float miny = amin(amin(amin(1e30f, p0y), p1y), p2y);
   float maxx = amax(amax(amax(-1e30f, p0x), p1x), p2x);
@@ -322,16 +335,17 @@ MR_HD inline bool setup_triangle(const float* P, bool persp, float w, float h,
   r.bx1 = (int)floorf(maxx);
   r.by0 = (int)floorf(miny);
   r.by1 = (int)floorf(maxy);
-  if (persp) {
-    r.z0 = -1.0f / a.z;
-    r.z1 = -1.0f / b.z;
-    r.z2 = -1.0f / c.z;
-  } else {
-    r.z0 = ndc0.z;
-    r.z1 = ndc1.z;
-    r.z2 = ndc2.z;
-  }
+  r.z0 = v0.zv;
+  r.z1 = v1.zv;
+  r.z2 = v2.zv;
   return true;
+}
+
+// Triangle setup (Renderer.cpp:179-236) for eye-space positions.
+MR_HD inline bool setup_triangle(const float* P, bool persp, float w, float h,
+                                 f3 a, f3 b, f3 c, TriSetup& r) {
+  return setup_screen(w, h, project_vertex(P, persp, w, h, a),
+                      project_vertex(P, persp, w, h, b), project_vertex(P, persp, w, h, c), r);
 }
 
 // fract + nearest texel index of Renderer.cpp:274, clamped to n-1 where the
