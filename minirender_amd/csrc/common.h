@@ -39,6 +39,7 @@ struct DrawDesc {
   int64_t tri_begin;    // first flat triangle index of this draw in the batch
   uint32_t order_base;  // 2 * triangles of earlier draws of the same job
   int32_t tiles_x;
+  int64_t vbase;        // first projected-vertex record of this draw's mesh
 };
 
 // One frame of a batch.
@@ -64,6 +65,10 @@ struct JobDesc {
   float* rgb;
   float* normals;
 };
+
+// Projected-vertex record written once per vertex by the vertex kernel:
+// eye-space position (Renderer.cpp:344-345) and its projection.
+constexpr int kVrecFloats = 6;  // ex, ey, ez, px, py, zv
 
 // A surviving (sub-)triangle as the binner sees it: its id (2 * flat
 // triangle index + clip sub-index), its draw and clamped pixel bbox.

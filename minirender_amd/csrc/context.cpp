@@ -13,7 +13,9 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <map>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "minirender_amd.h"
@@ -109,7 +111,7 @@ void slot_free(mr_ctx* c, Slot& s) {
   DevBuf* bufs[] = {&x.items, &x.region_count, &x.tile_count, &x.tile_off,
                     &x.tile_cursor, &x.tile_heavy, &x.tile_done, &x.parts, &x.bins,
                     &x.heavy_keys, &x.scan_status, &x.counters, &x.block_stats,
-                    &x.tables, &x.win, &x.outputs};
+                    &x.tables, &x.vrec, &x.win, &x.outputs};
   for (DevBuf* b : bufs) dev_free(c, *b, true);
   HostBuf* hbufs[] = {&x.h_tables, &x.h_counters, &x.h_outputs};
   for (HostBuf* b : hbufs) host_free(*b);
@@ -129,6 +131,9 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
   plan.draw_begin.clear();
   plan.job_bases.assign(2 * (size_t)nframes, 0);
   plan.total_tris = 0;
+  plan.vg_begin.clear();
+  plan.vg_draw.clear();
+  plan.total_verts = 0;
   plan.ntiles = 0;
   plan.npixels = 0;
   plan.shade_blocks = 0;
@@ -168,6 +173,7 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
     J.lighting = f->lighting != 0;
     J.save_normals = f->save_normals != 0;
     uint64_t order = 0;
+    std::map<std::string, int64_t> groups;  // (mesh, modelview) -> first vertex record
     for (int d = 0; d < f->ndraws; d++) {
       const mr_draw& s = f->draws[d];
       if (s.mesh < 0 || s.mesh >= (int)c->meshes.size() || !c->meshes[s.mesh].alive)
@@ -207,6 +213,20 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
       D.job = j;
       D.tile_base = J.tile_base;
       D.tiles_x = J.tiles_x;
+      {
+        std::string key((const char*)&s.mesh, sizeof(s.mesh));
+        key.append((const char*)s.modelview, 12 * sizeof(float));
+        auto it = groups.find(key);
+        if (it != groups.end()) {
+          D.vbase = it->second;
+        } else {
+          D.vbase = plan.total_verts;
+          groups.emplace(key, D.vbase);
+          plan.vg_begin.push_back(plan.total_verts);
+          plan.vg_draw.push_back((int32_t)plan.draws.size());
+          plan.total_verts += m.npos;
+        }
+      }
       D.tri_begin = plan.total_tris;
       D.order_base = (uint32_t)order;
       order += 2ull * (uint64_t)m.ntri;
@@ -217,7 +237,7 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
     if (order >= 0xFFFFFFFFull) return MR_ERR_ARG;
     J.draw_end = (int32_t)plan.draws.size();
   }
-  if (2 * plan.total_tris >= 0xFFFFFFFFll) return MR_ERR_ARG;
+  if (2 * plan.total_tris >= 0xFFFFFFFFll || plan.total_verts >= 0x7FFFFFFFll) return MR_ERR_ARG;
   auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
   TableLayout& L = plan.layout;
   const size_t nd = std::max<size_t>(plan.draws.size(), 1);
@@ -225,7 +245,10 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
   L.bases = up(plan.jobs.size() * sizeof(JobDesc));
   L.draws = L.bases + up(plan.job_bases.size() * 4);
   L.draw_begin = L.draws + up(nd * sizeof(DrawDesc));
-  L.bytes = L.draw_begin + up(nd * 8);
+  const size_t ng = std::max<size_t>(plan.vg_begin.size(), 1);
+  L.vg_begin = L.draw_begin + up(nd * 8);
+  L.vg_draw = L.vg_begin + up(ng * 8);
+  L.bytes = L.vg_draw + up(ng * 4);
   return MR_OK;
 }
 
@@ -253,6 +276,7 @@ static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins
   if ((rc = dev_alloc(c, s.counters, kNumCounters * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.block_stats, 2 * kStatSlots * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.tables, p.layout.bytes, true))) return rc;
+  if ((rc = dev_alloc(c, s.vrec, (size_t)p.total_verts * kVrecFloats * 4, true))) return rc;
   if ((rc = dev_alloc(c, s.win, (size_t)p.npixels * 4, true))) return rc;
   if ((rc = host_alloc(s.h_tables, p.layout.bytes))) return rc;
   if ((rc = host_alloc(s.h_counters, kNumCounters * 8))) return rc;
@@ -283,6 +307,11 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.counters = (unsigned long long*)s.counters.p;
   a.block_stats = (unsigned long long*)s.block_stats.p;
   char* tb = (char*)s.tables.p;
+  a.vrec = (float*)s.vrec.p;
+  a.total_verts = p.total_verts;
+  a.vg_begin = (const int64_t*)(tb + p.layout.vg_begin);
+  a.vg_draw = (const int32_t*)(tb + p.layout.vg_draw);
+  a.nvgroups = (int32_t)p.vg_begin.size();
   a.draws = (const DrawDesc*)(tb + p.layout.draws);
   a.draw_begin = (const int64_t*)(tb + p.layout.draw_begin);
   a.ndraws = (int32_t)p.draws.size();
@@ -306,6 +335,7 @@ static int launch_stages(const BatchArgs& a, Slot& s, bool profile, bool clear) 
   if (profile) HIP_TRY(hipEventRecord(s.ev[0], st));
   if (clear) HIP_TRY(launch_clear(a, st));
   if (profile) HIP_TRY(hipEventRecord(s.ev[1], st));
+  HIP_TRY(launch_vertex(a, st));
   HIP_TRY(launch_geometry(a, st));
   if (profile) HIP_TRY(hipEventRecord(s.ev[2], st));
   HIP_TRY(launch_scan(a, st));
@@ -345,6 +375,8 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   if (!p.draws.empty()) {
     memcpy(ht + p.layout.draws, p.draws.data(), p.draws.size() * sizeof(DrawDesc));
     memcpy(ht + p.layout.draw_begin, p.draw_begin.data(), p.draws.size() * 8);
+    memcpy(ht + p.layout.vg_begin, p.vg_begin.data(), p.vg_begin.size() * 8);
+    memcpy(ht + p.layout.vg_draw, p.vg_draw.data(), p.vg_draw.size() * 4);
   }
   const BatchArgs a = make_args(c, s, p);
   s.last_geom_grid = p.total_tris ? geom_grid(a) : 0;

@@ -44,7 +44,7 @@ struct TexRec {
 struct Scratch {
   DevBuf items, region_count, tile_count, tile_off, tile_cursor, tile_heavy,
       tile_done, parts, bins, heavy_keys, scan_status, counters, block_stats,
-      tables, win, outputs;
+      tables, vrec, win, outputs;
   // The batch tables (jobs, job tile/shade bases, draws, draw starts) sit in
   // one pinned staging buffer and one device buffer, uploaded by one copy.
   HostBuf h_tables, h_counters, h_outputs;
@@ -52,7 +52,7 @@ struct Scratch {
 
 // Byte offsets of the tables inside Scratch::tables.
 struct TableLayout {
-  size_t jobs = 0, bases = 0, draws = 0, draw_begin = 0, bytes = 0;
+  size_t jobs = 0, bases = 0, draws = 0, draw_begin = 0, vg_begin = 0, vg_draw = 0, bytes = 0;
 };
 
 // Jobs and draws of a batch, as the kernels see them.
@@ -61,6 +61,11 @@ struct BatchPlan {
   std::vector<DrawDesc> draws;
   std::vector<int64_t> draw_begin;
   std::vector<int32_t> job_bases;  // tile bases of all jobs, then shade bases
+  // Vertex groups: draws of one job with the same mesh and modelview share
+  // their projected vertices.
+  std::vector<int64_t> vg_begin;
+  std::vector<int32_t> vg_draw;
+  int64_t total_verts = 0;
   int64_t total_tris = 0;
   int64_t ntiles = 0;
   int64_t npixels = 0;

@@ -136,31 +136,46 @@ __device__ inline f3 load3(const float* p, int i) {
   return f3{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
 }
 
-// Eye-space corners of triangle t (Renderer.cpp:344-345, 364-366).
-__device__ inline void corners(const DrawDesc& D, int t, f3& a, f3& b, f3& c) {
+// Projected-vertex record of vertex vi of draw D (written by k_vertex).
+__device__ inline void load_vrec(const BatchArgs& a, const DrawDesc& D, int vi, f3& e,
+                                 ScreenVtx& v) {
+  const float2* r = reinterpret_cast<const float2*>(a.vrec + (D.vbase + vi) * kVrecFloats);
+  const float2 r0 = r[0], r1 = r[1], r2 = r[2];
+  e = f3{r0.x, r0.y, r1.x};
+  v = ScreenVtx{r1.y, r2.x, r2.y};
+}
+
+// Eye-space corners and projections of triangle t (Renderer.cpp:344-345,
+// 364-366).
+__device__ inline void corners(const BatchArgs& a, const DrawDesc& D, int t, f3 e[3],
+                               ScreenVtx v[3]) {
   const int* ip = D.idx + 3 * t;
-  a = xform(D.mv, load3(D.pos, ip[0]));
-  b = xform(D.mv, load3(D.pos, ip[1]));
-  c = xform(D.mv, load3(D.pos, ip[2]));
+  const int i0 = ip[0], i1 = ip[1], i2 = ip[2];
+  load_vrec(a, D, i0, e[0], v[0]);
+  load_vrec(a, D, i1, e[1], v[1]);
+  load_vrec(a, D, i2, e[2], v[2]);
 }
 
 // Near test, clip and setup of triangle t (Renderer.cpp:169-236).  Returns
 // the number of surviving sub-triangles (0..2) and their sub-indices.
-__device__ inline int tri_setup(const DrawDesc& D, const JobDesc& J, int t,
+// Unclipped triangles use the per-vertex projections; clipped ones project
+// their clipped corners here.
+__device__ inline int tri_setup(const BatchArgs& a, const DrawDesc& D, const JobDesc& J, int t,
                                 TriSetup& s0, int& sub0, TriSetup& s1,
                                 int& sub1, bool& clipped) {
-  f3 e0, e1, e2;
-  corners(D, t, e0, e1, e2);
+  f3 e[3];
+  ScreenVtx v[3];
+  corners(a, D, t, e, v);
   const float zn = J.znear;
   const bool persp = J.persp != 0;
   const float w = (float)J.W, h = (float)J.H;
-  const bool b0 = e0.z > zn, b1 = e1.z > zn, b2 = e2.z > zn;
+  const bool b0 = e[0].z > zn, b1 = e[1].z > zn, b2 = e[2].z > zn;
   clipped = false;
   if (b0 || b1 || b2) {
     if (b0 && b1 && b2) return 0;
     clipped = true;
     f3 a0, a1, a2, c0, c1, c2;
-    const int ns = clip_positions(zn, e0, e1, e2, a0, a1, a2, c0, c1, c2);
+    const int ns = clip_positions(zn, e[0], e[1], e[2], a0, a1, a2, c0, c1, c2);
     int n = 0;
     if (setup_triangle(J.P, persp, w, h, a0, a1, a2, s0)) {
       sub0 = 0;
@@ -175,45 +190,62 @@ __device__ inline int tri_setup(const DrawDesc& D, const JobDesc& J, int t,
     }
     return n;
   }
-  if (setup_triangle(J.P, persp, w, h, e0, e1, e2, s0)) {
+  if (setup_screen(w, h, v[0], v[1], v[2], s0)) {
     sub0 = 0;
     return 1;
   }
   return 0;
 }
 
-// Setup of sub-triangle `sub` of a binned triangle from its eye-space corners.
-__device__ inline void tri_setup_eye(const JobDesc& J, const f3& e0, const f3& e1, const f3& e2,
-                                     int sub, TriSetup& s) {
+// Setup of sub-triangle `sub` of a binned triangle from its corners.
+__device__ inline void tri_setup_corners(const JobDesc& J, const f3 e[3], const ScreenVtx v[3],
+                                         int sub, TriSetup& s) {
   const float zn = J.znear;
-  const bool persp = J.persp != 0;
   const float w = (float)J.W, h = (float)J.H;
-  if (e0.z > zn || e1.z > zn || e2.z > zn) {
+  if (e[0].z > zn || e[1].z > zn || e[2].z > zn) {
+    const bool persp = J.persp != 0;
     f3 a0, a1, a2, c0, c1, c2;
-    clip_positions(zn, e0, e1, e2, a0, a1, a2, c0, c1, c2);
+    clip_positions(zn, e[0], e[1], e[2], a0, a1, a2, c0, c1, c2);
     if (sub) setup_triangle(J.P, persp, w, h, c0, c1, c2, s);
     else setup_triangle(J.P, persp, w, h, a0, a1, a2, s);
     return;
   }
-  setup_triangle(J.P, persp, w, h, e0, e1, e2, s);
+  setup_screen(w, h, v[0], v[1], v[2], s);
 }
 
 // Setup of sub-triangle `sub` of a binned triangle (which survived setup).
-__device__ inline void tri_setup_sub(const DrawDesc& D, const JobDesc& J, int t,
-                                     int sub, TriSetup& s) {
-  f3 e0, e1, e2;
-  corners(D, t, e0, e1, e2);
-  const float zn = J.znear;
-  const bool persp = J.persp != 0;
-  const float w = (float)J.W, h = (float)J.H;
-  if (e0.z > zn || e1.z > zn || e2.z > zn) {
-    f3 a0, a1, a2, c0, c1, c2;
-    clip_positions(zn, e0, e1, e2, a0, a1, a2, c0, c1, c2);
-    if (sub) setup_triangle(J.P, persp, w, h, c0, c1, c2, s);
-    else setup_triangle(J.P, persp, w, h, a0, a1, a2, s);
-    return;
+__device__ inline void tri_setup_sub(const BatchArgs& a, const DrawDesc& D, const JobDesc& J,
+                                     int t, int sub, TriSetup& s) {
+  f3 e[3];
+  ScreenVtx v[3];
+  corners(a, D, t, e, v);
+  tri_setup_corners(J, e, v, sub, s);
+}
+
+// (a): one lane per vertex of every vertex group: eye-space position and
+// its projection (Renderer.cpp:340-349, 191-203), once per vertex instead of
+// once per triangle corner in each later kernel.
+__global__ __launch_bounds__(256) void k_vertex(BatchArgs a) {
+  for (int64_t base = (int64_t)blockIdx.x * 256; base < a.total_verts;
+       base += (int64_t)gridDim.x * 256) {
+    const int64_t wfirst = base + (threadIdx.x & ~63);
+    if (wfirst >= a.total_verts) continue;
+    const int64_t i = wfirst + (threadIdx.x & 63);
+    int gw = search_last(a.vg_begin, 0, a.nvgroups - 1,
+                         (int64_t)__builtin_amdgcn_readfirstlane((int)wfirst));
+    gw = __builtin_amdgcn_readfirstlane(gw);
+    if (i >= a.total_verts) continue;
+    int g = gw;
+    while (g + 1 < a.nvgroups && a.vg_begin[g + 1] <= i) g++;
+    const DrawDesc& D = a.draws[a.vg_draw[g]];
+    const JobDesc& J = a.jobs[D.job];
+    const f3 e = xform(D.mv, load3(D.pos, (int)(i - a.vg_begin[g])));
+    const ScreenVtx v = project_vertex(J.P, J.persp != 0, (float)J.W, (float)J.H, e);
+    float2* r = reinterpret_cast<float2*>(a.vrec + i * kVrecFloats);
+    r[0] = make_float2(e.x, e.y);
+    r[1] = make_float2(e.z, v.px);
+    r[2] = make_float2(v.py, v.zv);
   }
-  setup_triangle(J.P, persp, w, h, e0, e1, e2, s);
 }
 
 // For every active lane's tile rectangle, visit all (lane, tile) pairs with
@@ -316,7 +348,7 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
         while (d + 1 < a.ndraws && a.draw_begin[d + 1] <= g) d++;
       auto setup = [&](const DrawDesc& D) {
         const JobDesc& J = a.jobs[D.job];
-        ns = tri_setup(D, J, (int)(g - D.tri_begin), s0, sub0, s1, sub1, clipped);
+        ns = tri_setup(a, D, J, (int)(g - D.tri_begin), s0, sub0, s1, sub1, clipped);
         tbase = D.tile_base;
         tiles_x = D.tiles_x;
       };
@@ -585,10 +617,10 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
         const uint32_t d0 = __builtin_amdgcn_readfirstlane(be.y);
         if (__ballot(be.y == d0) == __ballot(1)) {
           const DrawDesc Du = uniform_draw(a, (int)d0);
-          tri_setup_sub(Du, J, (int)(g - Du.tri_begin), (int)(id & 1u), s);
+          tri_setup_sub(a, Du, J, (int)(g - Du.tri_begin), (int)(id & 1u), s);
         } else {
           const DrawDesc& D = a.draws[be.y];
-          tri_setup_sub(D, J, (int)(g - D.tri_begin), (int)(id & 1u), s);
+          tri_setup_sub(a, D, J, (int)(g - D.tri_begin), (int)(id & 1u), s);
         }
         nrows = min(s.by1, y0 + TH - 1) - max(s.by0, y0) + 1;
         setups[tid] = s;
@@ -694,7 +726,11 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
     t1 = it[1];
     t2 = it[2];
   }
-  const f3 P0 = load3(D.pos, i0), P1 = load3(D.pos, i1), P2 = load3(D.pos, i2);
+  f3 ec[3];
+  ScreenVtx vc[3];
+  load_vrec(a, D, i0, ec[0], vc[0]);
+  load_vrec(a, D, i1, ec[1], vc[1]);
+  load_vrec(a, D, i2, ec[2], vc[2]);
   const f3 N0 = load3(D.nrm, n0), N1 = load3(D.nrm, n1), N2 = load3(D.nrm, n2);
   f2 U0 = f2{0.0f, 0.0f}, U1 = U0, U2 = U0;  // Renderer.cpp:371-379
   if (D.tidx) {
@@ -704,11 +740,11 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   }
   // Eye-space vertices as paintMesh computes them (Renderer.cpp:344-349).
   Vtx v0, v1, v2;
-  v0.p = xform(D.mv, P0);
-  v1.p = xform(D.mv, P1);
-  v2.p = xform(D.mv, P2);
+  v0.p = ec[0];
+  v1.p = ec[1];
+  v2.p = ec[2];
   TriSetup s;
-  tri_setup_eye(J, v0.p, v1.p, v2.p, sub, s);
+  tri_setup_corners(J, ec, vc, sub, s);
   float e1, e2;
   edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
              (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
@@ -888,6 +924,14 @@ hipError_t launch_clear(const BatchArgs& a, hipStream_t s) {
   const int n = std::max(std::max(a.scan_blocks, a.ntiles), (int)kNumCounters);
   const int grid = std::min((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_clear, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_vertex(const BatchArgs& a, hipStream_t s) {
+  if (a.total_verts == 0) return hipSuccess;
+  const int64_t blocks = (a.total_verts + 255) / 256;
+  const int grid = (int)std::min<int64_t>(blocks, (int64_t)a.num_cus * 8);
+  hipLaunchKernelGGL(k_vertex, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

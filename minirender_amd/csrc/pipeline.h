@@ -30,6 +30,11 @@ struct BatchArgs {
   // serialize): [0, kStatSlots) clipped triangles per geometry workgroup,
   // [kStatSlots, 2 kStatSlots) binned triangles per bin workgroup.
   unsigned long long* block_stats;
+  float* vrec;                // [total_verts * kVrecFloats] projected vertices
+  int64_t total_verts;
+  const int64_t* vg_begin;    // [nvgroups] first record of each vertex group
+  const int32_t* vg_draw;     // [nvgroups] a draw of the group (mesh, modelview, job)
+  int32_t nvgroups;
   const DrawDesc* draws;
   const int64_t* draw_begin;  // [ndraws] tri_begin of each draw
   int32_t ndraws;
@@ -66,6 +71,7 @@ __host__ __device__ inline int32_t bin_grid(const BatchArgs& a) {
 }
 
 hipError_t launch_clear(const BatchArgs& a, hipStream_t s);
+hipError_t launch_vertex(const BatchArgs& a, hipStream_t s);
 hipError_t launch_geometry(const BatchArgs& a, hipStream_t s);
 hipError_t launch_scan(const BatchArgs& a, hipStream_t s);
 hipError_t launch_bin(const BatchArgs& a, hipStream_t s);
