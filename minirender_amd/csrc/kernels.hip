@@ -565,7 +565,10 @@ __device__ inline void raster_pixel(u64* keys, int p, float e1, float e2,
 template <int TW, int TH, int NT>
 __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
   constexpr int NP = TW * TH;
-  __shared__ u64 keys[NP];
+  // Key rows padded by one entry: pixels of one column in different rows
+  // fall in different LDS banks.
+  constexpr int KS = TW + 1;
+  __shared__ u64 keys[TH * KS];
   __shared__ TriSetup setups[NT];
   __shared__ uint32_t setup_id[NT];
   __shared__ int row_start[NT];
@@ -598,7 +601,7 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
     const int local = tile - J.tile_base;
     const int tyi = local / J.tiles_x, txi = local - tyi * J.tiles_x;
     const int x0 = txi * TW, y0 = tyi * TH;
-    for (int p = tid; p < NP; p += NT) keys[p] = ~0ull;
+    for (int p = tid; p < TH * KS; p += NT) keys[p] = ~0ull;
     const uint32_t off = a.tile_off[tile];
     uint32_t avail = cnt;  // entries actually written (bin-list overflow)
     if ((u64)off >= a.bins_cap) avail = 0;
@@ -656,7 +659,7 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
                    (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
         e1 = advance(e1, s.n1x, cx0 - s.bx0);
         e2 = advance(e2, s.n2x, cx0 - s.bx0);
-        const int prow = (row - y0) * TW - x0;
+        const int prow = (row - y0) * KS - x0;
         const uint32_t id = setup_id[i];
         for (int c = cx0; c <= cx1; c++) {
           raster_pixel(keys, prow + c, e1, e2, s, persp, id);
@@ -668,9 +671,9 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
     }
     if (nparts == 1) {
       for (int p = tid; p < NP; p += NT) {
-        const int c = x0 + (p % TW), row = y0 + (p / TW);
+        const int c = x0 + (p % TW), row = y0 + (p / TW), lp = (p / TW) * KS + (p % TW);
         if (c < J.W && row < J.H)
-          J.win[(size_t)row * J.W + c] = keys[p] == ~0ull ? kNoWinner : (uint32_t)keys[p];
+          J.win[(size_t)row * J.W + c] = keys[lp] == ~0ull ? kNoWinner : (uint32_t)keys[lp];
       }
     } else {
       // Split tile: merge into the tile's global key buffer; the last part
@@ -684,10 +687,11 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
       if (slot != kNoSlot) {
         u64* gk = a.heavy_keys + (u64)slot * NP;
         u64 seen = ~0ull;
-        for (int p = tid; p < NP; p += NT)
-          if (keys[p] != ~0ull)
-            seen &= __hip_atomic_fetch_min(&gk[p], keys[p], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
+        for (int p = tid; p < NP; p += NT) {
+          const u64 kp = keys[(p / TW) * KS + (p % TW)];
+          if (kp != ~0ull)
+            seen &= __hip_atomic_fetch_min(&gk[p], kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         if (seen == 0ull) sh_last = 0;  // consumes the returned values
         __syncthreads();
         if (tid == 0)
@@ -706,6 +710,11 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
     }
   }
 }
+
+// Reciprocal length and normalisation with the hardware reciprocal square
+// root (v_rsq_f32, about 1 ulp): for colour only.
+__device__ inline float fast_rlen3(f3 a) { return __builtin_amdgcn_rsqf(dot3(a, a)); }
+__device__ inline f3 fast_norm3(f3 a) { return scale3(fast_rlen3(a), a); }
 
 // Shading of a covered pixel whose winner `id` belongs to draw D.
 __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, const DrawDesc& D,
@@ -783,19 +792,23 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   }
   f3 value = f3{D.emissive[0], D.emissive[1], D.emissive[2]};
   if (J.lighting) {  // :279-304, FAST_LIGHT
+    // Colour only (the contract is 1 LSB of RGB8): hardware reciprocal,
+    // reciprocal square root and exp2/log2 replace the correctly rounded
+    // divide, sqrt and powf; the error is a few ulp of fp32.
     const f3 position = add3(add3(scale3(k[0], v0.p), scale3(k[1], v1.p)),
                              scale3(k[2], v2.p));
     const f3 L = f3{J.light[0], J.light[1], J.light[2]};
-    const f3 lightdir = J.light_is_point ? norm3(sub3(L, position)) : L;
+    const f3 lightdir = J.light_is_point ? fast_norm3(sub3(L, position)) : L;
     const f3 normal = add3(add3(scale3(k[0], v0.n), scale3(k[1], v1.n)),
                            scale3(k[2], v2.n));
-    const float diff = amax(0.0f, dot3(normal, lightdir)) / len3(normal) + J.ambient;
+    const float inl = fast_rlen3(normal);
+    const float diff = amax(0.0f, dot3(normal, lightdir)) * inl + J.ambient;
     value = add3(value, scale3(diff, color));
     if (D.shininess != 0.0f) {
-      const f3 view = norm3(position);
+      const f3 view = fast_norm3(position);
       const f3 hv = sub3(lightdir, view);
-      const float sp = powf(amax(dot3(hv, normal), 0.0f) / (len3(hv) * len3(normal)),
-                            D.shininess);
+      const float cosa = amax(dot3(hv, normal), 0.0f) * (fast_rlen3(hv) * inl);
+      const float sp = __powf(cosa, D.shininess);
       value = add3(value, scale3(sp, f3{D.specular[0], D.specular[1], D.specular[2]}));
     }
     if (J.normals && J.save_normals) {
