@@ -393,19 +393,33 @@ def serving_section(args, torch, mra):
             log(f"serving {where} {name}: {res[name]}")
         out[where] = res
     # Highest offered rate each mode sustains with p99 within the budget
-    # (device framebuffers): doubling sweep from 500 req/s.
+    # (device framebuffers): doubling from 500 req/s, then three bisections.
+    def sustains(mb, sl, rate):
+        count = int(min(n, max(2000, rate * 1.5)))
+        r = trace(mb, sl, arrivals(rate, count), count, "device")
+        r["rate"] = round(rate, 1)
+        ok = r["p99_ms"] <= args.p99_budget and r["frames_per_s"] >= 0.9 * rate
+        log(f"sweep {r}: {'ok' if ok else 'over'}")
+        return ok, r
+
     cap = {}
     for name, (mb, sl) in modes.items():
-        best, rate = None, 500.0
-        while rate <= 256000:
-            count = int(min(n, max(2000, rate * 1.5)))
-            r = trace(mb, sl, arrivals(rate, count), count, "device")
-            r["rate"] = rate
-            log(f"sweep {name} {rate:.0f}/s: {r}")
-            if r["p99_ms"] > args.p99_budget or r["frames_per_s"] < 0.9 * rate:
+        best, rate, fail = None, 500.0, None
+        while rate <= 512000:
+            ok, r = sustains(mb, sl, rate)
+            if not ok:
+                fail = rate
                 break
-            best = r
-            rate *= 2
+            best, rate = r, rate * 2
+        if best is not None and fail is not None:
+            lo, hi = best["rate"], fail
+            for _ in range(3):
+                mid = (lo + hi) / 2
+                ok, r = sustains(mb, sl, mid)
+                if ok:
+                    best, lo = r, mid
+                else:
+                    hi = mid
         cap[name] = best
     out["at_p99_budget"] = cap
     if cap.get("plain") and cap.get("batched"):
