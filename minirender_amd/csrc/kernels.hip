@@ -132,6 +132,17 @@ __device__ inline DrawDesc uniform_draw(const BatchArgs& a, int d) {
   return out;
 }
 
+// The same for a job descriptor indexed by a workgroup-uniform value.
+__device__ inline JobDesc uniform_job(const BatchArgs& a, int j) {
+  typedef const __attribute__((address_space(4))) uint32_t* ConstWords;
+  const ConstWords src = (ConstWords)(a.jobs + j);
+  JobDesc out;
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(JobDesc) / 4); i++) dst[i] = src[i];
+  return out;
+}
+
 __device__ inline f3 load3(const float* p, int i) {
   return f3{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
 }
@@ -235,16 +246,24 @@ __global__ __launch_bounds__(256) void k_vertex(BatchArgs a) {
                          (int64_t)__builtin_amdgcn_readfirstlane((int)wfirst));
     gw = __builtin_amdgcn_readfirstlane(gw);
     if (i >= a.total_verts) continue;
-    int g = gw;
-    while (g + 1 < a.nvgroups && a.vg_begin[g + 1] <= i) g++;
-    const DrawDesc& D = a.draws[a.vg_draw[g]];
-    const JobDesc& J = a.jobs[D.job];
-    const f3 e = xform(D.mv, load3(D.pos, (int)(i - a.vg_begin[g])));
-    const ScreenVtx v = project_vertex(J.P, J.persp != 0, (float)J.W, (float)J.H, e);
-    float2* r = reinterpret_cast<float2*>(a.vrec + i * kVrecFloats);
-    r[0] = make_float2(e.x, e.y);
-    r[1] = make_float2(e.z, v.px);
-    r[2] = make_float2(v.py, v.zv);
+    const int64_t gw_end = gw + 1 < a.nvgroups ? a.vg_begin[gw + 1] : 0x7FFFFFFFFFFFFFFFll;
+    auto emit = [&](const DrawDesc& D, const JobDesc& J, int64_t gbegin) {
+      const f3 e = xform(D.mv, load3(D.pos, (int)(i - gbegin)));
+      const ScreenVtx v = project_vertex(J.P, J.persp != 0, (float)J.W, (float)J.H, e);
+      float2* r = reinterpret_cast<float2*>(a.vrec + i * kVrecFloats);
+      r[0] = make_float2(e.x, e.y);
+      r[1] = make_float2(e.z, v.px);
+      r[2] = make_float2(v.py, v.zv);
+    };
+    if (__ballot(i < gw_end) == __ballot(1)) {  // one group: descriptors via scalar loads
+      const DrawDesc Du = uniform_draw(a, __builtin_amdgcn_readfirstlane(a.vg_draw[gw]));
+      emit(Du, uniform_job(a, __builtin_amdgcn_readfirstlane(Du.job)), a.vg_begin[gw]);
+    } else {
+      int g = gw;
+      while (g + 1 < a.nvgroups && a.vg_begin[g + 1] <= i) g++;
+      const DrawDesc& D = a.draws[a.vg_draw[g]];
+      emit(D, a.jobs[D.job], a.vg_begin[g]);
+    }
   }
 }
 
@@ -346,14 +365,17 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
       int d = dw;
       if (g >= next_begin)
         while (d + 1 < a.ndraws && a.draw_begin[d + 1] <= g) d++;
-      auto setup = [&](const DrawDesc& D) {
-        const JobDesc& J = a.jobs[D.job];
+      auto setup = [&](const DrawDesc& D, const JobDesc& J) {
         ns = tri_setup(a, D, J, (int)(g - D.tri_begin), s0, sub0, s1, sub1, clipped);
         tbase = D.tile_base;
         tiles_x = D.tiles_x;
       };
-      if (__ballot(d == dw) == __ballot(1)) setup(uniform_draw(a, dw));
-      else setup(a.draws[d]);
+      if (__ballot(d == dw) == __ballot(1)) {
+        const DrawDesc Du = uniform_draw(a, dw);
+        setup(Du, uniform_job(a, __builtin_amdgcn_readfirstlane(Du.job)));
+      } else {
+        setup(a.draws[d], a.jobs[a.draws[d].job]);
+      }
       draw = (uint32_t)d;
     }
     nclipped += (uint32_t)__popcll(__ballot(clipped));
@@ -595,8 +617,8 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
     const uint32_t cnt = a.tile_count[(size_t)tile * kCtrStride];
     const uint32_t nparts = cnt <= a.frag_size ? 1u : (cnt + a.frag_size - 1) / a.frag_size;
     if (b >= nparts_all && nparts > 1) return;  // rendered through its parts
-    const int j = search_last(a.job_tile_base, 0, a.njobs - 1, tile);
-    const JobDesc& J = a.jobs[j];
+    const int j = __builtin_amdgcn_readfirstlane(search_last(a.job_tile_base, 0, a.njobs - 1, tile));
+    const JobDesc J = uniform_job(a, j);
     const bool persp = J.persp != 0;
     const int local = tile - J.tile_base;
     const int tyi = local / J.tiles_x, txi = local - tyi * J.tiles_x;
@@ -907,8 +929,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
   __shared__ int64_t dcache[kMaxLdsDraws];
   const int tid = threadIdx.x;
   const int b = (int)xcd_remap(blockIdx.x, a.shade_blocks);
-  const int j = search_last(a.job_shade_base, 0, a.njobs - 1, b);
-  const JobDesc& J = a.jobs[j];
+  const int j = __builtin_amdgcn_readfirstlane(search_last(a.job_shade_base, 0, a.njobs - 1, b));
+  const JobDesc J = uniform_job(a, j);
   const int ncached = fill_draw_cache(dcache, a, J, tid, kShadeBlock);
   // The raster and scan are done with the tile counters, part counters and
   // look-back words: leave them zero for the next batch on this scratch.
