@@ -1,18 +1,35 @@
-// io.h -- image files of the minirender API (reference:
-// include/minirender/io.h:10-22): PPM (P6) save with the reference's RGB8
-// quantization and load with texel = byte / 255.  Mesh and scene file
-// loaders (STL / OBJ / X3D) are not part of this library.
+// io.h -- files of the minirender API (reference: include/minirender/io.h:
+// 10-22): PPM (P6) save with the reference's RGB8 quantization and load with
+// texel = byte / 255; STL (ASCII / binary) and OBJ (+ MTL) loaders, STL and
+// XYZ point writers.  X3D scenes are not read (loadMesh returns an empty
+// node for them).
 #ifndef MINIRENDER_IO_H
 #define MINIRENDER_IO_H
 
 #include <string>
+#include <vector>
 
+#include "Scene.h"
 #include "vecmath.h"
 
 namespace minirender {
 
 bool savePPM(const Array2<Vec3>& image, const std::string& filename);
 Array2<Vec3> loadPPM(const std::string& filename);
+
+// io.cpp:114-133: .stl -> a node holding the mesh, .obj -> loadOBJ.
+Shared<SceneNode> loadMesh(const std::string& filename);
+// io.cpp:135-152 (binary when the size is 84 + 50 * facets).
+Shared<TriMesh> loadSTL(const std::string& filename);
+// io.cpp:154-181.
+bool saveSTL(const Shared<TriMesh>& mesh, const std::string& name);
+// io.cpp:185-335: one mesh per material, fan-triangulated faces.
+Shared<SceneNode> loadOBJ(const std::string& filename);
+// io.cpp:417-431: points with z < -1e-5, transformed by m, "%f %f %f" lines.
+bool saveXYZ(const Array2<Vec3>& points, const std::string& filename,
+             const Matrix4& m = Matrix4::identity());
+// x3d.cpp:17-33: fan triangulation of -1-terminated polygons.
+std::vector<int> triangulateIndices(const std::vector<int>& indices);
 
 }  // namespace minirender
 

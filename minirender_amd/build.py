@@ -28,8 +28,8 @@ COMMON = [
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
     "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
 ]
-SOURCES = ["kernels.hip", "context.cpp", "projection.cpp", "cpp_api.cpp", "server.cpp",
-           "tune.cpp"]
+SOURCES = ["kernels.hip", "context.cpp", "projection.cpp", "cpp_api.cpp", "io.cpp",
+           "server.cpp", "tune.cpp"]
 
 
 def _compile(src: Path) -> Path:

@@ -32,6 +32,82 @@ using namespace minirender;
     }                                         \
   } while (0)
 
+// STL / OBJ loaders on small files written here, against the reference's
+// parsing rules (io.cpp:15-335, x3d.cpp:17-33).
+static int loader_checks() {
+  // fan triangulation with -1 separators
+  const std::vector<int> fan = triangulateIndices({0, 1, 2, 3, -1, 4, 5, 6, -1});
+  const std::vector<int> fan_ref = {0, 1, 2, 0, 2, 3, 4, 5, 6};
+  CHECK(fan == fan_ref, "triangulateIndices");
+  // OBJ: a quad and a triangle with "v//vn" in material "b", a triangle
+  // without material (""), a material "a" never used; vt v flipped.
+  {
+    FILE* m = std::fopen("/tmp/mr_api_check.mtl", "w");
+    std::fprintf(m, "newmtl a\nKd 0.1 0.2 0.3\nnewmtl b\nKd 0.5 0.6 0.7\nNs 0\nKs 1 1 1\n");
+    std::fclose(m);
+    FILE* o = std::fopen("/tmp/mr_api_check.obj", "w");
+    std::fprintf(o,
+                 "# test\nmtllib mr_api_check.mtl\n"
+                 "v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nv 2 2 2\n"
+                 "vt 0.25 0.25\nvn 0 0 1\n"
+                 "f 1/1/1 2/1/1 3/1/1\n"
+                 "usemtl b\nf 1/1/1 2/1/1 3/1/1 4/1/1\nf 1//1 3//1 5//1\n");
+    std::fclose(o);
+    Shared<SceneNode> node = loadOBJ("/tmp/mr_api_check.obj");
+    CHECK(node && node->children.size() == 2, "OBJ meshes");
+    auto* m0 = dynamic_cast<TriMesh*>(node->children[0].get());
+    auto* m1 = dynamic_cast<TriMesh*>(node->children[1].get());
+    CHECK(m0 && m1, "OBJ mesh types");
+    CHECK(m0->indices == std::vector<int>({0, 1, 2}), "OBJ default mesh indices");
+    CHECK(m1->indices == std::vector<int>({0, 1, 2, 0, 2, 3, 0, 2, 4}), "OBJ quad + tri");
+    // "v//vn" leaves texcoord index -1, which closes the polygon: only the
+    // quad keeps texcoord indices
+    CHECK(m1->texcoordsI == std::vector<int>({0, 0, 0, 0, 0, 0}), "OBJ texcoord indices");
+    CHECK(m1->normalsI.size() == 9 && m1->normals.size() == 1, "OBJ normals");
+    CHECK(m1->texcoords.size() == 1 && m1->texcoords[0].y == 0.75f, "OBJ vt flip");
+    CHECK(m1->material && m1->material->diffuse.x == 0.5f && m1->material->shininess == 10.0f,
+          "OBJ material b");
+    CHECK(m0->material && m0->material->diffuse.x == 0.7f, "OBJ default material");
+    CHECK(m0->vertices.size() == 5, "OBJ shared vertices");
+  }
+  // OBJ without normals: one normal per face
+  {
+    FILE* o = std::fopen("/tmp/mr_api_check2.obj", "w");
+    std::fprintf(o, "v 0 0 0\nv 2 0 0\nv 0 2 0\nf 1 2 3\n");
+    std::fclose(o);
+    Shared<SceneNode> node = loadOBJ("/tmp/mr_api_check2.obj");
+    auto* m = dynamic_cast<TriMesh*>(node->children[0].get());
+    CHECK(m && m->normals.size() == 1 && m->normals[0].z == 1.0f &&
+              m->normalsI == std::vector<int>({0, 0, 0}),
+          "OBJ face normals");
+  }
+  // STL: binary round trip of a cube, then an ASCII file
+  {
+    Shared<TriMesh> cube = createCube(1.0f);
+    CHECK(saveSTL(cube, "/tmp/mr_api_check.stl"), "saveSTL");
+    Shared<TriMesh> back = loadSTL("/tmp/mr_api_check.stl");
+    CHECK(back && back->indices.size() == cube->indices.size() &&
+              back->vertices.size() == cube->indices.size() && back->normals.size() == 12,
+          "STL binary round trip sizes");
+    for (size_t i = 0; i < cube->indices.size(); i++) {
+      const Vec3 a = cube->vertices[cube->indices[i]], b = back->vertices[back->indices[i]];
+      CHECK(a.x == b.x && a.y == b.y && a.z == b.z, "STL vertex %zu", i);
+    }
+    FILE* s = std::fopen("/tmp/mr_api_check_a.stl", "w");
+    std::fprintf(s,
+                 "solid t\nfacet normal 0 0 1\nouter loop\nvertex 0 0 0\nvertex 1 0 0\n"
+                 "vertex 0 1 0\nendloop\nendfacet\nendsolid t\n");
+    std::fclose(s);
+    Shared<TriMesh> a = loadSTL("/tmp/mr_api_check_a.stl");
+    CHECK(a && a->indices == std::vector<int>({0, 1, 2}) && a->normals.size() == 1 &&
+              a->normalsI == std::vector<int>({0, 0, 0}) && a->vertices[1].x == 1.0f,
+          "STL ascii");
+    Shared<SceneNode> node = loadMesh("/tmp/mr_api_check_a.stl");
+    CHECK(node && node->children.size() == 1, "loadMesh stl");
+  }
+  return 0;
+}
+
 static int cpu_checks() {
   // createSphere(r, lat, long): (lat-1)*long + 2 vertices, 2*long*(lat-1) triangles
   auto s = createSphere(1.0f, 32, 64);
@@ -88,6 +164,7 @@ static int cpu_checks() {
   Array2<Vec3> back = loadPPM(path);
   CHECK(back.rows() == 3 && back.cols() == 4, "loadPPM size");
   CHECK(back(0, 0).z == 127.0f / 255.0f && back(2, 3).x == 1.0f, "PPM values");
+  if (int rc = loader_checks()) return rc;
   std::printf("ok\n");
   return 0;
 }
