@@ -80,6 +80,8 @@ typedef struct mr_outputs {
                      /* (0/1); 0xFFFFFFFF where nothing was drawn                 */
   float* rgb;        /* H*W*3 unquantized colour, as Renderer::getImage (:383)    */
   float* normals;    /* H*W*3 eye-space normals, as getNormalsImage (h:63)        */
+  float* points;     /* H*W*3 eye-space points from the depth buffer, as          */
+                     /* Renderer::getRangeImage (Renderer.cpp:388-415)           */
   int32_t on_device;
 } mr_outputs;
 

@@ -20,7 +20,7 @@ from .scenes import Draw, FrameSpec, Material, Mesh, Texture
 __all__ = ["Context", "FrameSpec", "Draw", "Material", "Mesh", "Texture", "MrError",
            "NO_WINNER", "scenes", "frame_struct", "load_library", "OUTPUT_KINDS"]
 
-OUTPUT_KINDS = ("rgb8", "depth", "winner", "rgb", "normals")
+OUTPUT_KINDS = ("rgb8", "depth", "winner", "rgb", "normals", "points")
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -67,7 +67,7 @@ def alloc_outputs(frame: FrameSpec, kinds: Iterable[str]) -> Dict[str, np.ndarra
     h, w = frame.height, frame.width
     shapes = {"rgb8": ((h, w, 3), np.uint8), "depth": ((h, w), np.float32),
               "winner": ((h, w), np.uint32), "rgb": ((h, w, 3), np.float32),
-              "normals": ((h, w, 3), np.float32)}
+              "normals": ((h, w, 3), np.float32), "points": ((h, w, 3), np.float32)}
     return {k: np.empty(*shapes[k]) for k in kinds}
 
 

@@ -51,7 +51,8 @@ class mr_frame(C.Structure):
 class mr_outputs(C.Structure):
     _fields_ = [
         ("rgb8", C.c_void_p), ("depth", C.c_void_p), ("winner", C.c_void_p),
-        ("rgb", C.c_void_p), ("normals", C.c_void_p), ("on_device", C.c_int32),
+        ("rgb", C.c_void_p), ("normals", C.c_void_p), ("points", C.c_void_p),
+        ("on_device", C.c_int32),
     ]
 
 

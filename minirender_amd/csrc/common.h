@@ -64,6 +64,7 @@ struct JobDesc {
   uint32_t* winner;
   float* rgb;
   float* normals;
+  float* points;       // range image (Renderer.cpp:388-415)
 };
 
 // Projected-vertex record written once per vertex by the vertex kernel:

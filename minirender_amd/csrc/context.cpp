@@ -500,19 +500,19 @@ int finish_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   return MR_OK;
 }
 
-static const size_t kOutBytes[5] = {3, 4, 4, 12, 12};  // rgb8 depth winner rgb normals
+static const size_t kOutBytes[kOutKinds] = {3, 4, 4, 12, 12, 12};  // rgb8 depth winner rgb normals points
 
 int bind_outputs(mr_ctx* c, Slot& s, BatchPlan& p, const mr_outputs* outs,
                  std::vector<size_t>& offs) {
   const int n = (int)p.jobs.size();
   size_t total = 0;
-  offs.assign((size_t)n * 5, 0);
+  offs.assign((size_t)n * kOutKinds, 0);
   for (int j = 0; j < n; j++) {
     const mr_outputs& o = outs[j];
     const size_t npix = (size_t)p.jobs[j].W * p.jobs[j].H;
-    const void* ptrs[5] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals};
-    for (int k = 0; k < 5; k++) {
-      offs[(size_t)j * 5 + k] = total;
+    const void* ptrs[kOutKinds] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals, o.points};
+    for (int k = 0; k < kOutKinds; k++) {
+      offs[(size_t)j * kOutKinds + k] = total;
       if (ptrs[k] && !o.on_device) total += (npix * kOutBytes[k] + 255) & ~(size_t)255;
     }
   }
@@ -524,12 +524,15 @@ int bind_outputs(mr_ctx* c, Slot& s, BatchPlan& p, const mr_outputs* outs,
     JobDesc& J = p.jobs[j];
     if (o.on_device) {
       J.rgb8 = o.rgb8; J.depth = o.depth; J.winner = o.winner; J.rgb = o.rgb; J.normals = o.normals;
+      J.points = o.points;
     } else {
-      J.rgb8 = o.rgb8 ? (uint8_t*)(base + offs[(size_t)j * 5 + 0]) : nullptr;
-      J.depth = o.depth ? (float*)(base + offs[(size_t)j * 5 + 1]) : nullptr;
-      J.winner = o.winner ? (uint32_t*)(base + offs[(size_t)j * 5 + 2]) : nullptr;
-      J.rgb = o.rgb ? (float*)(base + offs[(size_t)j * 5 + 3]) : nullptr;
-      J.normals = o.normals ? (float*)(base + offs[(size_t)j * 5 + 4]) : nullptr;
+      const size_t* of = &offs[(size_t)j * kOutKinds];
+      J.rgb8 = o.rgb8 ? (uint8_t*)(base + of[0]) : nullptr;
+      J.depth = o.depth ? (float*)(base + of[1]) : nullptr;
+      J.winner = o.winner ? (uint32_t*)(base + of[2]) : nullptr;
+      J.rgb = o.rgb ? (float*)(base + of[3]) : nullptr;
+      J.normals = o.normals ? (float*)(base + of[4]) : nullptr;
+      J.points = o.points ? (float*)(base + of[5]) : nullptr;
     }
   }
   return MR_OK;
@@ -543,10 +546,10 @@ int copy_outputs(mr_ctx* c, Slot& s, const BatchPlan& p, const mr_outputs* outs,
     const mr_outputs& o = outs[j];
     if (o.on_device) continue;
     const size_t npix = (size_t)p.jobs[j].W * p.jobs[j].H;
-    void* dst[5] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals};
-    for (int k = 0; k < 5; k++) {
+    void* dst[kOutKinds] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals, o.points};
+    for (int k = 0; k < kOutKinds; k++) {
       if (!dst[k]) continue;
-      HIP_TRY(hipMemcpyAsync(dst[k], (char*)s.sc.outputs.p + offs[j * 5 + k], npix * kOutBytes[k],
+      HIP_TRY(hipMemcpyAsync(dst[k], (char*)s.sc.outputs.p + offs[j * kOutKinds + k], npix * kOutBytes[k],
                              hipMemcpyDeviceToHost, s.stream));
       copied = true;
     }

@@ -13,6 +13,7 @@
 namespace mr {
 
 constexpr size_t kGraphCache = 32;
+constexpr int kOutKinds = 6;  // rgb8, depth, winner, rgb, normals, points (mr_outputs order)
 
 struct DevBuf {
   void* p = nullptr;

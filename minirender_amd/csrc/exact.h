@@ -348,6 +348,26 @@ MR_HD inline bool setup_triangle(const float* P, bool persp, float w, float h,
                       project_vertex(P, persp, w, h, b), project_vertex(P, persp, w, h, c), r);
 }
 
+// Range image point of pixel (i = row, j = column) from its depth
+// (Renderer.cpp:388-415): back-projection through the projection's diagonal
+// terms, (0,0,0) past the far plane.
+MR_HD inline float range_far_depth(const float* P) {
+  const bool persp = P[15] == 0.0f;
+  const float zfar = persp ? P[11] / (P[10] + 1) : (P[11] - 1) / P[10];
+  return persp ? zfar : 1.0f;
+}
+
+MR_HD inline f3 range_point(const float* P, float fardepth, float w, float h, int i, int j,
+                            float depth) {
+  if (depth > fardepth) return f3{0.0f, 0.0f, 0.0f};
+  const float u = (j + 0.5f) / (w / 2) - 1;
+  const float v = -(i + 0.5f) / (h / 2) + 1;
+  const float z = -depth;
+  const float x = -(u + P[2]) * z / P[0];
+  const float y = -(v + P[6]) * z / P[5];
+  return f3{x, y, z};
+}
+
 // fract + nearest texel index of Renderer.cpp:274, clamped to n-1 where the
 // reference would read one past the end (fract rounding to 1.0f).
 MR_HD inline int tex_index(float u, int n) {

@@ -733,6 +733,14 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
   }
 }
 
+// Range image of the pixel from its depth (Renderer.cpp:388-415).
+__device__ inline void store_point(const JobDesc& J, size_t pix, int c, int row, float depth) {
+  const f3 q = range_point(J.P, range_far_depth(J.P), (float)J.W, (float)J.H, row, c, depth);
+  J.points[pix * 3 + 0] = q.x;
+  J.points[pix * 3 + 1] = q.y;
+  J.points[pix * 3 + 2] = q.z;
+}
+
 // Reciprocal length and normalisation with the hardware reciprocal square
 // root (v_rsq_f32, about 1 ulp): for colour only.
 __device__ inline float fast_rlen3(f3 a) { return __builtin_amdgcn_rsqf(dot3(a, a)); }
@@ -787,6 +795,7 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   if (!J.rgb8 && !J.rgb && !(J.normals && J.save_normals)) {
     if (J.depth) J.depth[pix] = z;
     if (J.winner) J.winner[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
+    if (J.points) store_point(J, pix, c, row, z);
     return;
   }
 
@@ -855,6 +864,7 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   }
   if (J.depth) J.depth[pix] = z;
   if (J.winner) J.winner[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
+  if (J.points) store_point(J, pix, c, row, z);
 }
 
 // Deferred shading of one pixel from its winner (Renderer.cpp:271-305).
@@ -880,6 +890,7 @@ __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
       J.normals[pix * 3 + 1] = 0.0f;
       J.normals[pix * 3 + 2] = 1.0f;
     }
+    if (J.points) store_point(J, pix, c, row, kClearDepth);
     return;
   }
   const int d = lookup_draw(dcache, ncached, a, J, (int64_t)(id >> 1));

@@ -67,6 +67,7 @@ uint64_t request_bytes(mr_ctx* c, const mr_frame& f, const mr_outputs& o) {
   if (o.winner) b += npix * 4;
   if (o.rgb) b += npix * 12;
   if (o.normals) b += npix * 12;
+  if (o.points) b += npix * 12;
   uint64_t tris = 0;
   for (int d = 0; d < f.ndraws; d++) {
     const int h = f.draws[d].mesh;
@@ -114,24 +115,24 @@ static bool is_pinned(const void* p) {
   return at.type == hipMemoryTypeHost;
 }
 
-static const size_t kOutBytes[5] = {3, 4, 4, 12, 12};  // rgb8 depth winner rgb normals
+static const size_t kOutBytes[kOutKinds] = {3, 4, 4, 12, 12, 12};
 
 // Queue the read-back of every host output of the batch on the slot's
 // stream: pinned destinations directly, the rest into pinned staging (copied
 // out by complete()).  Marks which outputs need the staging copy.
 int mr_server::queue_readback(Flight& f) {
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  f.staged.assign(f.reqs.size() * 5, 0);
+  f.staged.assign(f.reqs.size() * kOutKinds, 0);
   size_t need = 0;
   for (size_t j = 0; j < f.reqs.size(); j++) {
     const mr_outputs& o = f.reqs[j]->out;
     if (o.on_device) continue;
     const size_t npix = (size_t)f.plan.jobs[j].W * f.plan.jobs[j].H;
-    const void* dst[5] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals};
-    for (int k = 0; k < 5; k++)
+    const void* dst[kOutKinds] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals, o.points};
+    for (int k = 0; k < kOutKinds; k++)
       if (dst[k] && !is_pinned(dst[k])) {
-        f.staged[j * 5 + k] = 1;
-        need = std::max(need, f.offs[j * 5 + k] + npix * kOutBytes[k]);
+        f.staged[j * kOutKinds + k] = 1;
+        need = std::max(need, f.offs[j * kOutKinds + k] + npix * kOutBytes[k]);
       }
   }
   if (need && host_alloc(f.slot.sc.h_outputs, need)) return MR_ERR_NOMEM;
@@ -139,11 +140,11 @@ int mr_server::queue_readback(Flight& f) {
     const mr_outputs& o = f.reqs[j]->out;
     if (o.on_device) continue;
     const size_t npix = (size_t)f.plan.jobs[j].W * f.plan.jobs[j].H;
-    void* dst[5] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals};
-    for (int k = 0; k < 5; k++) {
+    void* dst[kOutKinds] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals, o.points};
+    for (int k = 0; k < kOutKinds; k++) {
       if (!dst[k]) continue;
-      const size_t off = f.offs[j * 5 + k];
-      void* to = f.staged[j * 5 + k] ? (char*)f.slot.sc.h_outputs.p + off : dst[k];
+      const size_t off = f.offs[j * kOutKinds + k];
+      void* to = f.staged[j * kOutKinds + k] ? (char*)f.slot.sc.h_outputs.p + off : dst[k];
       if (hipMemcpyAsync(to, (const char*)f.slot.sc.outputs.p + off, npix * kOutBytes[k],
                          hipMemcpyDeviceToHost, f.slot.stream) != hipSuccess) {
         (void)hipGetLastError();
@@ -214,10 +215,10 @@ void mr_server::complete(Flight& f) {
     const mr_outputs& o = f.reqs[j]->out;
     if (o.on_device) continue;
     const size_t npix = (size_t)f.plan.jobs[j].W * f.plan.jobs[j].H;
-    void* dst[5] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals};
-    for (int k = 0; k < 5; k++)
-      if (dst[k] && f.staged[j * 5 + k])
-        memcpy(dst[k], (const char*)f.slot.sc.h_outputs.p + f.offs[j * 5 + k],
+    void* dst[kOutKinds] = {o.rgb8, o.depth, o.winner, o.rgb, o.normals, o.points};
+    for (int k = 0; k < kOutKinds; k++)
+      if (dst[k] && f.staged[j * kOutKinds + k])
+        memcpy(dst[k], (const char*)f.slot.sc.h_outputs.p + f.offs[j * kOutKinds + k],
                npix * kOutBytes[k]);
   }
   std::lock_guard<std::mutex> lk(mu);

@@ -324,6 +324,32 @@ static void set_material(ostate* s, const mr_draw* d, const mro_texture* tex,
 }
 
 /* Renderer::render, Renderer.cpp:311-332, with paintMesh :334-381. */
+/* Renderer::getRangeImage (Renderer.cpp:388-415): each pixel's eye-space
+ * point back-projected from the depth buffer through the projection's
+ * diagonal terms; (0,0,0) where the depth lies past the far depth. */
+static void range_image(const mr_frame* f, const float* depth, float* pts) {
+  const float* P = f->projection;
+  const int persp = P[15] == 0.0f;
+  const float zfar = persp ? P[11] / (P[10] + 1) : (P[11] - 1) / P[10];
+  const float fardepth = persp ? zfar : 1.0f;
+  const float w = (float)f->width, h = (float)f->height;
+  for (int i = 0; i < f->height; i++)
+    for (int j = 0; j < f->width; j++) {
+      const size_t k = (size_t)i * f->width + j;
+      float x = 0.0f, y = 0.0f, z = 0.0f;
+      if (!(depth[k] > fardepth)) {
+        const float u = (j + 0.5f) / (w / 2) - 1;
+        const float v = -(i + 0.5f) / (h / 2) + 1;
+        z = -depth[k];
+        x = -(u + P[2]) * z / P[0];
+        y = -(v + P[6]) * z / P[5];
+      }
+      pts[k * 3 + 0] = x;
+      pts[k * 3 + 1] = y;
+      pts[k * 3 + 2] = z;
+    }
+}
+
 int mro_render(const mr_frame* f, const mro_mesh* meshes, int32_t nmeshes,
                const mro_texture* textures, int32_t ntextures,
                const mr_outputs* out) {
@@ -414,6 +440,7 @@ int mro_render(const mr_frame* f, const mro_mesh* meshes, int32_t nmeshes,
 
   if (rc == MR_OK && out->rgb8)
     for (size_t i = 0; i < npix * 3; i++) out->rgb8[i] = quantize(image[i]);
+  if (rc == MR_OK && out->points) range_image(f, depth, out->points);
   if (!out->rgb) free(image);
   if (!out->depth) free(depth);
   return rc;

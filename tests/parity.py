@@ -2,7 +2,7 @@
 
 Bar: winner (draw-order index of the visible triangle) and depth bits equal,
 RGB8 colour within 1 LSB per channel, unquantized colour within 1e-5 relative
-(powf of libm vs the device library), normals bit-equal.
+(powf of libm vs the device library), normals and range image bit-equal.
 """
 from __future__ import annotations
 
@@ -37,6 +37,9 @@ def assert_parity(gpu: dict, ora: dict, what: str = "", normals: bool = True) ->
         both_nan = np.isnan(g) & np.isnan(o)
         close = np.isclose(g, o, rtol=1e-5, atol=1e-6) | both_nan
         assert close.all(), f"{what}: float colour differs at {np.argwhere(~close)[0]}"
+    if "points" in gpu:
+        assert np.array_equal(gpu["points"].view(np.uint32), ora["points"].view(np.uint32)), \
+            f"{what}: range image differs at {np.argwhere(gpu['points'] != ora['points'])[:1]}"
     if "normals" in gpu and normals:
         assert np.array_equal(gpu["normals"].view(np.uint32), ora["normals"].view(np.uint32)), \
             f"{what}: normals differ"

@@ -17,7 +17,7 @@ from tests.parity import assert_parity, fuzz_frame, stack_frame
 
 pytestmark = pytest.mark.gpu
 
-ALL = ("rgb8", "depth", "winner", "rgb", "normals")
+ALL = ("rgb8", "depth", "winner", "rgb", "normals", "points")
 
 
 def check(ctx, frame, what, kinds=ALL):

@@ -108,6 +108,39 @@ def test_perspective_correct_depth():
     assert worst < 2e-5, worst
 
 
+def test_range_image_is_the_ray_plane_hit():
+    """getRangeImage (Renderer.cpp:388-415): each covered pixel's point is the
+    eye-space hit of its pixel-centre ray on the tilted plane (a plugged-in
+    sign or index mistake moves it off the plane); uncovered pixels are 0."""
+    fr, proj, _ = tilted_plane_frame()
+    o = oracle_py.render(fr, ("depth", "winner", "points"))
+    cov = o["winner"] != 0xFFFFFFFF
+    pts = o["points"].astype(np.float64)
+    assert (pts[~cov] == 0).all()
+    worst = 0.0
+    for i, j in np.argwhere(cov):
+        p = plane_hit(pixel_ray(proj.astype(np.float64), 64, 48, i, j))
+        worst = max(worst, np.abs(pts[i, j] - p).max() / np.abs(p).max())
+    assert worst < 5e-5, worst
+
+
+def test_range_image_orthographic_keeps_pixel_grid():
+    """Orthographic: the far test uses depth 1 (NDC), x/y are the pixel
+    centres scaled by the projection (Renderer.cpp:390-393, 408-409)."""
+    proj = scenes.projection_ortho(-2, 2, -1.5, 1.5, 0.5, 10.0)
+    pos = np.array([[-1, -1, -3], [1, -1, -3], [1, 1, -3], [-1, 1, -3]], np.float32)
+    nrm = np.tile(np.array([[0, 0, 1]], np.float32), (4, 1))
+    fr = FrameSpec(40, 30, proj, draws=[Draw(Mesh(pos, nrm, np.array([[0, 1, 2], [0, 2, 3]], np.int32)))])
+    o = oracle_py.render(fr, ("depth", "winner", "points"))
+    cov = o["winner"] != 0xFFFFFFFF
+    i, j = np.argwhere(cov)[0]
+    u = (j + 0.5) / 20 - 1
+    d = float(o["depth"][i, j])  # NDC z of the plane
+    assert abs(o["points"][i, j, 0] - (u * d / proj[0, 0])) < 1e-6
+    assert abs(o["points"][i, j, 2] + d) < 1e-7
+    assert (o["points"][~cov] == 0).all()
+
+
 def test_lambert_directional_closed_form():
     fr, proj, n = tilted_plane_frame(light=(0.3, 0.4, 1.0))
     o = oracle_py.render(fr, ("rgb", "winner"))
