@@ -42,7 +42,7 @@ sys.path.insert(0, ROOT)
 METRIC = "Mtris/s + frames/s at 1080p; p50/p99 ms/request, batched vs plain GPU"
 HBM_PEAK_GBS = 8000.0
 STAGES = ["clear", "geometry", "scan", "bin", "raster", "shade"]
-KERNEL_OF = {"clear": "k_clear", "geometry": "k_geom", "scan": "k_scan", "bin": "k_bin",
+KERNEL_OF = {"clear": "k_clear", "geometry": "k_vertex+k_geom", "scan": "k_scan", "bin": "k_bin",
              "raster": "k_tile", "shade": "k_shade"}
 
 
@@ -122,16 +122,19 @@ def texel_bytes(frame, winner):
 
 
 def kernel_alg_bytes(frame):
-    """Algorithmic HBM bytes per launch of each kernel for one frame."""
+    """Algorithmic HBM bytes per launch of each stage for one frame: each
+    part of the frame's bytes is charged to the stage that must move it."""
     npx = frame.width * frame.height
-    geo = input_bytes(frame, with_attrs=False)
+    pos = sum(12 * len(d.mesh.positions) for d in frame.draws)
+    idx = sum(12 * d.mesh.ntri for d in frame.draws)
+    attrs = input_bytes(frame, True) - pos - idx
     return {
         "clear": 0,
-        "geometry": geo,       # positions + position indices
+        "geometry": pos + idx,  # vertex kernel (positions) + geometry kernel (indices)
         "scan": 0,
-        "bin": geo,            # the same inputs, re-read to emit (tile, triangle) pairs
-        "raster": 0,           # reads only the bin list (implementation bytes)
-        "shade": npx * (3 + 4) + texel_bytes(frame, None),  # RGB8 + depth out (+ texels)
+        "bin": 0,               # items and pairs are implementation bytes
+        "raster": 0,            # reads only the bin list (implementation bytes)
+        "shade": npx * (3 + 4) + attrs + texel_bytes(frame, None),  # outputs, normals, uvs, texels
     }
 
 
