@@ -237,15 +237,21 @@ __device__ inline void tri_setup_sub(const BatchArgs& a, const DrawDesc& D, cons
 // its projection (Renderer.cpp:340-349, 191-203), once per vertex instead of
 // once per triangle corner in each later kernel.
 __global__ __launch_bounds__(256) void k_vertex(BatchArgs a) {
-  for (int64_t base = (int64_t)blockIdx.x * 256; base < a.total_verts;
-       base += (int64_t)gridDim.x * 256) {
+  // Contiguous range per workgroup: one group search, then stepping.
+  const int64_t chunk = ((a.total_verts + (int64_t)gridDim.x * 256 - 1) /
+                         ((int64_t)gridDim.x * 256)) * 256;
+  const int64_t r_begin = (int64_t)blockIdx.x * chunk;
+  const int64_t r_end = min(r_begin + chunk, a.total_verts);
+  int gw = search_last(a.vg_begin, 0, a.nvgroups - 1,
+                       (int64_t)__builtin_amdgcn_readfirstlane((int)min(r_begin, a.total_verts)));
+  gw = __builtin_amdgcn_readfirstlane(gw);
+  for (int64_t base = r_begin; base < r_end; base += 256) {
     const int64_t wfirst = base + (threadIdx.x & ~63);
-    if (wfirst >= a.total_verts) continue;
+    if (wfirst >= r_end) continue;
     const int64_t i = wfirst + (threadIdx.x & 63);
-    int gw = search_last(a.vg_begin, 0, a.nvgroups - 1,
-                         (int64_t)__builtin_amdgcn_readfirstlane((int)wfirst));
+    while (gw + 1 < a.nvgroups && a.vg_begin[gw + 1] <= wfirst) gw++;
     gw = __builtin_amdgcn_readfirstlane(gw);
-    if (i >= a.total_verts) continue;
+    if (i >= r_end) continue;
     const int64_t gw_end = gw + 1 < a.nvgroups ? a.vg_begin[gw + 1] : 0x7FFFFFFFFFFFFFFFll;
     auto emit = [&](const DrawDesc& D, const JobDesc& J, int64_t gbegin) {
       const f3 e = xform(D.mv, load3(D.pos, (int)(i - gbegin)));
@@ -342,10 +348,19 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
   const int lane = threadIdx.x & 63;
   const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
   uint32_t nclipped = 0;
-  for (int64_t base = (int64_t)blockIdx.x * 256; base < a.total_tris;
-       base += (int64_t)gridDim.x * 256) {
+  // Each workgroup takes one contiguous range of triangles (a multiple of
+  // 256), so a wavefront finds its draw once and then only steps over draw
+  // boundaries; its draw and job descriptors are read with scalar loads.
+  const int64_t chunk = ((a.total_tris + (int64_t)gridDim.x * 256 - 1) /
+                         ((int64_t)gridDim.x * 256)) * 256;
+  const int64_t r_begin = (int64_t)blockIdx.x * chunk;
+  const int64_t r_end = min(r_begin + chunk, a.total_tris);
+  int dw = search_last(a.draw_begin, 0, a.ndraws - 1,
+                       (int64_t)__builtin_amdgcn_readfirstlane((int)min(r_begin, a.total_tris)));
+  dw = __builtin_amdgcn_readfirstlane(dw);
+  for (int64_t base = r_begin; base < r_end; base += 256) {
     const int64_t wfirst = base + (threadIdx.x & ~63);
-    if (wfirst >= a.total_tris) continue;
+    if (wfirst >= r_end) continue;
     const int64_t g = wfirst + lane;
     TriSetup s0, s1;
     s0.bx0 = s0.bx1 = s0.by0 = s0.by1 = 0;
@@ -354,14 +369,13 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
     bool clipped = false;
     uint32_t draw = 0;
     int tbase = 0, tiles_x = 1;
-    // The wave's draw: one scalar search for its first triangle (a uniform
-    // value, so the loads are scalar), then lanes past the next draw's start
-    // step forward (draw boundaries inside a wavefront are rare).
-    int dw = search_last(a.draw_begin, 0, a.ndraws - 1,
-                         (int64_t)__builtin_amdgcn_readfirstlane((int)wfirst));
+    // The wave's draw: step the uniform draw index to the wave's first
+    // triangle; lanes past the next draw's start step further (draw
+    // boundaries inside a wavefront are rare).
+    while (dw + 1 < a.ndraws && a.draw_begin[dw + 1] <= wfirst) dw++;
     dw = __builtin_amdgcn_readfirstlane(dw);
     const int64_t next_begin = dw + 1 < a.ndraws ? a.draw_begin[dw + 1] : 0x7FFFFFFFFFFFFFFFll;
-    if (g < a.total_tris) {
+    if (g < r_end) {
       int d = dw;
       if (g >= next_begin)
         while (d + 1 < a.ndraws && a.draw_begin[d + 1] <= g) d++;
@@ -617,8 +631,8 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
     const uint32_t cnt = a.tile_count[(size_t)tile * kCtrStride];
     const uint32_t nparts = cnt <= a.frag_size ? 1u : (cnt + a.frag_size - 1) / a.frag_size;
     if (b >= nparts_all && nparts > 1) return;  // rendered through its parts
-    const int j = __builtin_amdgcn_readfirstlane(search_last(a.job_tile_base, 0, a.njobs - 1, tile));
-    const JobDesc J = uniform_job(a, j);
+    const int j = search_last(a.job_tile_base, 0, a.njobs - 1, tile);
+    const JobDesc& J = a.jobs[j];
     const bool persp = J.persp != 0;
     const int local = tile - J.tile_base;
     const int tyi = local / J.tiles_x, txi = local - tyi * J.tiles_x;
