@@ -71,30 +71,13 @@ struct JobDesc {
 // eye-space position (Renderer.cpp:344-345) and its projection.
 constexpr int kVrecFloats = 6;  // ex, ey, ez, px, py, zv
 
-// A surviving (sub-)triangle as the binner sees it: its record index, its
-// draw and clamped pixel bbox.
+// A surviving (sub-)triangle as the binner sees it: its id (2 * flat
+// triangle index + clip sub-index), its draw and clamped pixel bbox.
 struct BinItem {
-  uint32_t rec;
+  uint32_t id;
   uint32_t draw;
   uint16_t bx0, bx1, by0, by1;
 };
-
-// The setup of a surviving (sub-)triangle, written once by the geometry
-// kernel and read by the raster and shading kernels (64 B, four 16-byte
-// loads).  Records are indexed region * kRegion + position, which orders
-// them like their ids (2 * flat triangle index + clip sub-index), so a
-// record index is a valid draw-order tie-break.
-struct TriRec {
-  float n1x, n1y, n2x, n2y;
-  float p0x, p0y, p2x, p2y;
-  float z0, z1, z2;
-  uint32_t id;
-  uint32_t draw;
-  uint32_t bx;  // bx0 | bx1 << 16
-  uint32_t by;  // by0 | by1 << 16
-  uint32_t pad;
-};
-static_assert(sizeof(TriRec) == 64, "TriRec must be 64 bytes");
 static_assert(sizeof(BinItem) == 16, "BinItem must be 16 bytes");
 
 // Device counters of a batch (64-bit, cleared by the clear kernel).
@@ -131,6 +114,7 @@ constexpr int kRegion = 128;       // item slots per geometry wavefront (64 x 2)
 constexpr int kShadeBlock = 256;   // pixels per deferred-shading workgroup
 constexpr int kShadeW = 64;        // shading block: kShadeW x kShadeH pixels
 constexpr int kShadeH = kShadeBlock / kShadeW;
+constexpr int kMaxLdsDraws = 256;  // draw-table entries cached in LDS
 
 // Bounded waits (decoupled look-back): polls before giving up with kError.
 constexpr uint32_t kSpinLimit = 1u << 22;

@@ -108,7 +108,7 @@ int slot_init(Slot& s, hipStream_t external) {
 void slot_free(mr_ctx* c, Slot& s) {
   if (s.stream) hipStreamSynchronize(s.stream);
   Scratch& x = s.sc;
-  DevBuf* bufs[] = {&x.items, &x.recs, &x.region_count, &x.tile_count, &x.tile_off,
+  DevBuf* bufs[] = {&x.items, &x.region_count, &x.tile_count, &x.tile_off,
                     &x.tile_cursor, &x.tile_heavy, &x.tile_done, &x.parts, &x.bins,
                     &x.heavy_keys, &x.scan_status, &x.counters, &x.block_stats,
                     &x.tables, &x.vrec, &x.win, &x.outputs};
@@ -263,7 +263,6 @@ static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins
   const size_t parts = (size_t)(bins_cap / c->frag_size) + heavy_slots + 16;
   int rc;
   if ((rc = dev_alloc(c, s.items, (size_t)nregions * kRegion * sizeof(BinItem), true))) return rc;
-  if ((rc = dev_alloc(c, s.recs, (size_t)nregions * kRegion * sizeof(TriRec), true))) return rc;
   if ((rc = dev_alloc(c, s.region_count, (size_t)nregions * 4, true))) return rc;
   if ((rc = dev_alloc(c, s.tile_count, nt * kCtrStride, true))) return rc;
   if ((rc = dev_alloc(c, s.tile_off, nt, true))) return rc;
@@ -271,7 +270,7 @@ static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins
   if ((rc = dev_alloc(c, s.tile_heavy, nt, true))) return rc;
   if ((rc = dev_alloc(c, s.tile_done, nt, true))) return rc;
   if ((rc = dev_alloc(c, s.parts, parts * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.bins, (size_t)bins_cap * 4, true))) return rc;
+  if ((rc = dev_alloc(c, s.bins, (size_t)bins_cap * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.heavy_keys, (size_t)heavy_slots * npt, true))) return rc;
   if ((rc = dev_alloc(c, s.scan_status, (size_t)std::max(scb, 1) * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.counters, kNumCounters * 8, true))) return rc;
@@ -298,9 +297,8 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.tile_done = (uint32_t*)s.tile_done.p;
   a.parts = (uint2*)s.parts.p;
   a.parts_cap = (uint32_t)std::min<size_t>(s.parts.bytes / 8, 0xFFFFFFF0u);
-  a.recs = (TriRec*)s.recs.p;
-  a.bins = (uint32_t*)s.bins.p;
-  a.bins_cap = s.bins.bytes / 4;
+  a.bins = (uint2*)s.bins.p;
+  a.bins_cap = s.bins.bytes / 8;
   if (a.bins_cap > 0xFFFFFFF0ull) a.bins_cap = 0xFFFFFFF0ull;
   a.heavy_keys = (unsigned long long*)s.heavy_keys.p;
   a.heavy_cap = (uint32_t)(s.heavy_keys.bytes / ((size_t)c->tile_w * c->tile_h * 8));
@@ -354,7 +352,7 @@ static int launch_stages(const BatchArgs& a, Slot& s, bool profile, bool clear) 
 static uint64_t bins_wanted(mr_ctx* c, const Slot& s, const BatchPlan& p) {
   uint64_t want = std::max<uint64_t>(
       (uint64_t)((double)p.total_tris * c->bin_factor) + (uint64_t)p.ntiles, 1u << 16);
-  return std::max<uint64_t>(want, s.sc.bins.bytes / 4);
+  return std::max<uint64_t>(want, s.sc.bins.bytes / 8);
 }
 
 int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
@@ -470,7 +468,7 @@ int finish_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
     int rc;
     if (hc[kBinOverflow]) {
       const uint64_t need = hc[kBinEntries] + hc[kBinEntries] / 4 + 1024;
-      if ((rc = dev_alloc(c, x.bins, (size_t)need * 4, true))) return rc;
+      if ((rc = dev_alloc(c, x.bins, (size_t)need * 8, true))) return rc;
     }
     if (hc[kHeavyOverflow]) {
       s.heavy_slots = (uint32_t)(hc[kHeavy] + hc[kHeavy] / 4 + 16);

@@ -43,7 +43,7 @@ struct TexRec {
 
 // Device scratch of one batch in flight, and its pinned staging.
 struct Scratch {
-  DevBuf items, recs, region_count, tile_count, tile_off, tile_cursor, tile_heavy,
+  DevBuf items, region_count, tile_count, tile_off, tile_cursor, tile_heavy,
       tile_done, parts, bins, heavy_keys, scan_status, counters, block_stats,
       tables, vrec, win, outputs;
   // The batch tables (jobs, job tile/shade bases, draws, draw starts) sit in

@@ -103,6 +103,23 @@ __device__ inline int search_last(const T* begin, int lo, int hi, int64_t x) {
   return lo;
 }
 
+// Draw of flat triangle g in job J: the job's draw table cached in LDS when
+// it fits (ncached > 0), else searched in global memory.
+__device__ inline int lookup_draw(const int64_t* dcache, int ncached,
+                                  const BatchArgs& a, const JobDesc& J, int64_t g) {
+  if (ncached > 0) return J.draw_begin + search_last(dcache, 0, ncached - 1, g);
+  return search_last(a.draw_begin, J.draw_begin, J.draw_end - 1, g);
+}
+
+// Fill the LDS draw cache for job J; returns the number of cached entries.
+__device__ inline int fill_draw_cache(int64_t* dcache, const BatchArgs& a,
+                                      const JobDesc& J, int tid, int nt) {
+  const int nd = J.draw_end - J.draw_begin;
+  const int n = nd <= kMaxLdsDraws ? nd : 0;
+  for (int i = tid; i < n; i += nt) dcache[i] = a.draw_begin[J.draw_begin + i];
+  return n;
+}
+
 // A draw descriptor read through the constant address space: for a
 // wavefront-uniform index its words are fetched with scalar loads (into
 // SGPRs) instead of one vector load per field.
@@ -192,6 +209,31 @@ __device__ inline int tri_setup(const BatchArgs& a, const DrawDesc& D, const Job
   return 0;
 }
 
+// Setup of sub-triangle `sub` of a binned triangle from its corners.
+__device__ inline void tri_setup_corners(const JobDesc& J, const f3 e[3], const ScreenVtx v[3],
+                                         int sub, TriSetup& s) {
+  const float zn = J.znear;
+  const float w = (float)J.W, h = (float)J.H;
+  if (e[0].z > zn || e[1].z > zn || e[2].z > zn) {
+    const bool persp = J.persp != 0;
+    f3 a0, a1, a2, c0, c1, c2;
+    clip_positions(zn, e[0], e[1], e[2], a0, a1, a2, c0, c1, c2);
+    if (sub) setup_triangle(J.P, persp, w, h, c0, c1, c2, s);
+    else setup_triangle(J.P, persp, w, h, a0, a1, a2, s);
+    return;
+  }
+  setup_screen(w, h, v[0], v[1], v[2], s);
+}
+
+// Setup of sub-triangle `sub` of a binned triangle (which survived setup).
+__device__ inline void tri_setup_sub(const BatchArgs& a, const DrawDesc& D, const JobDesc& J,
+                                     int t, int sub, TriSetup& s) {
+  f3 e[3];
+  ScreenVtx v[3];
+  corners(a, D, t, e, v);
+  tri_setup_corners(J, e, v, sub, s);
+}
+
 // (a): one lane per vertex of every vertex group: eye-space position and
 // its projection (Renderer.cpp:340-349, 191-203), once per vertex instead of
 // once per triangle corner in each later kernel.
@@ -262,50 +304,12 @@ __device__ inline uint32_t wave_rank_add(bool ok, int key, uint32_t* ctr) {
 // (taken in the counting pass), so the binner writes them without atomics.
 constexpr uint16_t kSingleTile = 0x8000;
 
-__device__ inline TriRec make_rec(const TriSetup& t, uint32_t id, uint32_t draw) {
-  TriRec r;
-  r.n1x = t.n1x; r.n1y = t.n1y; r.n2x = t.n2x; r.n2y = t.n2y;
-  r.p0x = t.p0x; r.p0y = t.p0y; r.p2x = t.p2x; r.p2y = t.p2y;
-  r.z0 = t.z0; r.z1 = t.z1; r.z2 = t.z2;
-  r.id = id;
-  r.draw = draw;
-  r.bx = (uint32_t)t.bx0 | ((uint32_t)t.bx1 << 16);
-  r.by = (uint32_t)t.by0 | ((uint32_t)t.by1 << 16);
-  r.pad = 0;
-  return r;
-}
-
-__device__ inline TriRec load_rec(const BatchArgs& a, uint32_t rec) {
-  const float4* src = reinterpret_cast<const float4*>(a.recs + rec);
-  const float4 q0 = src[0], q1 = src[1], q2 = src[2], q3 = src[3];
-  TriRec r;
-  r.n1x = q0.x; r.n1y = q0.y; r.n2x = q0.z; r.n2y = q0.w;
-  r.p0x = q1.x; r.p0y = q1.y; r.p2x = q1.z; r.p2y = q1.w;
-  r.z0 = q2.x; r.z1 = q2.y; r.z2 = q2.z;
-  r.id = __float_as_uint(q2.w);
-  r.draw = __float_as_uint(q3.x);
-  r.bx = __float_as_uint(q3.y);
-  r.by = __float_as_uint(q3.z);
-  r.pad = 0;
-  return r;
-}
-
-__device__ inline TriSetup rec_setup(const TriRec& r) {
-  TriSetup t;
-  t.n1x = r.n1x; t.n1y = r.n1y; t.n2x = r.n2x; t.n2y = r.n2y;
-  t.p0x = r.p0x; t.p0y = r.p0y; t.p2x = r.p2x; t.p2y = r.p2y;
-  t.z0 = r.z0; t.z1 = r.z1; t.z2 = r.z2;
-  t.bx0 = (int)(r.bx & 0xFFFFu); t.bx1 = (int)(r.bx >> 16);
-  t.by0 = (int)(r.by & 0xFFFFu); t.by1 = (int)(r.by >> 16);
-  return t;
-}
-
 // For every active lane's tile rectangle, visit all (lane, tile) pairs with
 // the whole wavefront (64 pairs per step, owners found by binary search of
 // the wave prefix sum).  Lanes with equal tiles form a group (at most
 // kMaxGroups groups per step; further lanes are groups of one) whose leader
 // calls op(tile, group size) once.  kScatter: op returns the first bin slot
-// of the group and every lane writes its record index `id` behind it.
+// of the group and every lane writes its (id, draw) pair behind it.
 template <bool kScatter, typename Op>
 __device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id, uint32_t draw,
                                 int tbase, int tiles_x, int tx0, int tx1,
@@ -356,11 +360,11 @@ __device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id, ui
     uint32_t old = 0;
     if (ok && lane == leader) old = op(key, gsize);
     if (kScatter) {
-      const uint32_t oid = __shfl(id, lo);
+      const uint32_t oid = __shfl(id, lo), odraw = __shfl(draw, lo);
       old = __shfl(old, leader);
       if (ok) {
         const uint32_t slot = old + rank;
-        if ((u64)slot < a.bins_cap) a.bins[slot] = oid;
+        if ((u64)slot < a.bins_cap) a.bins[slot] = make_uint2(oid, odraw);
         else atomicOr(&a.counters[kBinOverflow], 1ull);
       }
     }
@@ -442,12 +446,11 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
                              count);
       if (!act) return;
       if (single)
-        a.items[p] = BinItem{(uint32_t)p, draw, (uint16_t)(t.bx0 | kSingleTile),
+        a.items[p] = BinItem{id, draw, (uint16_t)(t.bx0 | kSingleTile),
                              (uint16_t)(slot & 0xFFFFu), (uint16_t)t.by0, (uint16_t)(slot >> 16)};
       else
-        a.items[p] = BinItem{(uint32_t)p, draw, (uint16_t)t.bx0, (uint16_t)t.bx1,
+        a.items[p] = BinItem{id, draw, (uint16_t)t.bx0, (uint16_t)t.bx1,
                              (uint16_t)t.by0, (uint16_t)t.by1};
-      a.recs[p] = make_rec(t, id, draw);
     };
     bin_sub(ns >= 1, s0, id0, pos);
     if (b2) bin_sub(ns == 2, s1, id1, pos + 1);
@@ -610,11 +613,12 @@ __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
       const bool single = act && (it.bx0 & kSingleTile);
       if (single) {  // slot taken in the counting pass
         const int key = D.tile_base + (it.by0 >> sh) * D.tiles_x + ((it.bx0 & 0x7FFF) >> sw);
-        const uint64_t slot = (uint64_t)a.tile_off[key] + ((uint32_t)it.bx1 | ((uint32_t)it.by1 << 16));
-        if (slot < a.bins_cap) a.bins[slot] = it.rec;
+        const uint64_t slot =
+            (uint64_t)a.tile_off[key] + ((uint32_t)it.bx1 | ((uint32_t)it.by1 << 16));
+        if (slot < a.bins_cap) a.bins[slot] = make_uint2(it.id, it.draw);
         else atomicOr(&a.counters[kBinOverflow], 1ull);
       }
-      wave_tile_pairs<true>(a, act && !single, it.rec, it.draw, D.tile_base, D.tiles_x,
+      wave_tile_pairs<true>(a, act && !single, it.id, it.draw, D.tile_base, D.tiles_x,
                             it.bx0 >> sw, it.bx1 >> sw, it.by0 >> sh, it.by1 >> sh, reserve);
     }
     r = rn;
@@ -696,13 +700,21 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
       // One lane per entry: setup, and the tile rows it covers.
       int nrows = 0;
       if (base + tid < e_end) {
-        // The triangle's setup from its record; the record index is the
-        // draw-order tie-break of the depth key.
-        const uint32_t rec = a.bins[off + base + tid];
-        const TriSetup s = rec_setup(load_rec(a, rec));
+        const uint2 be = a.bins[off + base + tid];
+        const uint32_t id = be.x;
+        const int64_t g = id >> 1;
+        TriSetup s;
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(be.y);
+        if (__ballot(be.y == d0) == __ballot(1)) {
+          const DrawDesc Du = uniform_draw(a, (int)d0);
+          tri_setup_sub(a, Du, J, (int)(g - Du.tri_begin), (int)(id & 1u), s);
+        } else {
+          const DrawDesc& D = a.draws[be.y];
+          tri_setup_sub(a, D, J, (int)(g - D.tri_begin), (int)(id & 1u), s);
+        }
         nrows = min(s.by1, y0 + TH - 1) - max(s.by0, y0) + 1;
         setups[tid] = s;
-        setup_id[tid] = rec;
+        setup_id[tid] = id;
       }
       // Exclusive scan of the row counts over the workgroup.
       int incl = nrows;
@@ -801,8 +813,7 @@ __device__ inline f3 fast_norm3(f3 a) { return scale3(fast_rlen3(a), a); }
 
 // Shading of a covered pixel whose winner `id` belongs to draw D.
 __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, const DrawDesc& D,
-                                          const TriRec& R, int c, int row, size_t pix) {
-  const uint32_t id = R.id;
+                                          uint32_t id, int c, int row, size_t pix) {
   const int64_t g = id >> 1;
   const int sub = (int)(id & 1u);
   const int tri = (int)(g - D.tri_begin);
@@ -836,7 +847,8 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   v0.p = ec[0];
   v1.p = ec[1];
   v2.p = ec[2];
-  const TriSetup s = rec_setup(R);
+  TriSetup s;
+  tri_setup_corners(J, ec, vc, sub, s);
   float e1, e2;
   edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
              (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
@@ -921,7 +933,9 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
 }
 
 // Deferred shading of one pixel from its winner (Renderer.cpp:271-305).
-__device__ void shade_pixel(const BatchArgs& a, const JobDesc& J, uint32_t id, int c, int row) {
+__device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
+                            const int64_t* dcache, int ncached, uint32_t id,
+                            int c, int row) {
   const size_t pix = (size_t)row * (size_t)J.W + (size_t)c;
   if (id == kNoWinner) {  // clear(), Renderer.cpp:113-119
     if (J.rgb8) {
@@ -944,15 +958,14 @@ __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J, uint32_t id, i
     if (J.points) store_point(J, pix, c, row, kClearDepth);
     return;
   }
-  const TriRec R = load_rec(a, id);  // the winner is a record index
-  const int d = (int)R.draw;
+  const int d = lookup_draw(dcache, ncached, a, J, (int64_t)(id >> 1));
   const int d0 = __builtin_amdgcn_readfirstlane(d);
   if (__ballot(d == d0) == __ballot(1)) {
     // One draw for the whole wavefront: its descriptor through scalar loads.
     const DrawDesc Du = uniform_draw(a, d0);
-    shade_lit(a, J, Du, R, c, row, pix);
+    shade_lit(a, J, Du, id, c, row, pix);
   } else {
-    shade_lit(a, J, a.draws[d], R, c, row, pix);
+    shade_lit(a, J, a.draws[d], id, c, row, pix);
   }
 }
 
@@ -989,10 +1002,12 @@ __device__ void reduce_stats(const BatchArgs& a, int tid) {
 // (e): one lane per pixel; a workgroup shades a kShadeW x kShadeH block of
 // one job (each wavefront one kShadeW-pixel row segment).
 __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
+  __shared__ int64_t dcache[kMaxLdsDraws];
   const int tid = threadIdx.x;
   const int b = (int)xcd_remap(blockIdx.x, a.shade_blocks);
   const int j = __builtin_amdgcn_readfirstlane(search_last(a.job_shade_base, 0, a.njobs - 1, b));
   const JobDesc J = uniform_job(a, j);
+  const int ncached = fill_draw_cache(dcache, a, J, tid, kShadeBlock);
   // The raster and scan are done with the tile counters, part counters and
   // look-back words: leave them zero for the next batch on this scratch.
   for (int64_t i = (int64_t)blockIdx.x * kShadeBlock + tid; i < a.ntiles || i < a.scan_blocks;
@@ -1012,7 +1027,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
   const int r = tid / kShadeW;
   const int c = (local - byi * bxn) * kShadeW + (tid - r * kShadeW), row = byi * kShadeH + r;
   if (c < J.W && row < J.H)
-    shade_pixel(a, J, J.win[(size_t)row * J.W + c], c, row);
+    shade_pixel(a, J, dcache, ncached, J.win[(size_t)row * J.W + c], c, row);
 }
 
 // ---- launchers -------------------------------------------------------------

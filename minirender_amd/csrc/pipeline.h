@@ -19,8 +19,7 @@ struct BatchArgs {
   uint32_t* tile_done;       // [ntiles] finished parts of a split tile
   uint2* parts;              // [parts_cap] (tile, part) of split tiles
   uint32_t parts_cap;
-  TriRec* recs;              // [nregions * kRegion] setups of surviving triangles
-  uint32_t* bins;            // [bins_cap] record indices grouped by tile
+  uint2* bins;               // [bins_cap] (triangle id, draw) grouped by tile
   uint64_t bins_cap;
   unsigned long long* heavy_keys;  // [heavy_cap * tile pixels] merge buffers
   uint32_t heavy_cap;
