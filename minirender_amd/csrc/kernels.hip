@@ -87,7 +87,6 @@ __global__ void k_clear(BatchArgs a) {
     if (i < a.scan_blocks) a.scan_status[i] = 0;
     if (i < a.ntiles) {
       a.tile_count[(size_t)i * kCtrStride] = 0;
-      a.tile_count[(size_t)i * kCtrStride + 1] = 0;
       a.tile_done[i] = 0;
     }
   }
@@ -274,36 +273,6 @@ __global__ __launch_bounds__(256) void k_vertex(BatchArgs a) {
   }
 }
 
-// Per-lane slot from one grouped atomic add per distinct key: lanes with
-// equal keys (at most 8 groups, further lanes alone) share one returning
-// atomicAdd of the group size at ctr[key * kCtrStride]; each lane gets the
-// returned base plus its rank in the group.
-__device__ inline uint32_t wave_rank_add(bool ok, int key, uint32_t* ctr) {
-  const int lane = threadIdx.x & 63;
-  u64 pend = __ballot(ok);
-  int leader = lane;
-  uint32_t rank = 0, gsize = 1;
-  for (int it = 0; it < 8 && pend; it++) {
-    const int l = __builtin_ctzll(pend);
-    const int kk = __builtin_amdgcn_readlane(key, l);
-    const u64 grp = pend & __ballot(key == kk);
-    if ((grp >> lane) & 1ull) {
-      leader = l;
-      rank = (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
-    }
-    if (lane == l) gsize = (uint32_t)__popcll(grp);
-    pend &= ~grp;
-  }
-  uint32_t old = 0;
-  if (ok && lane == leader) old = atomicAdd(&ctr[(size_t)key * kCtrStride], gsize);
-  old = __shfl(old, leader);
-  return old + rank;
-}
-
-// Items covering a single tile carry their slot within the tile's bin list
-// (taken in the counting pass), so the binner writes them without atomics.
-constexpr uint16_t kSingleTile = 0x8000;
-
 // For every active lane's tile rectangle, visit all (lane, tile) pairs with
 // the whole wavefront (64 pairs per step, owners found by binary search of
 // the wave prefix sum).  Lanes with equal tiles form a group (at most
@@ -431,29 +400,21 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
     const u64 lt = (1ull << lane) - 1ull;
     const int64_t pos = region * kRegion + __popcll(b1 & lt) + __popcll(b2 & lt);
     const uint32_t id0 = (uint32_t)(2 * g + sub0), id1 = (uint32_t)(2 * g + sub1);
-    // Per-tile counts: word 0 counts single-tile items (their returned slot
-    // is kept in the item), word 1 the pairs of multi-tile items.
+    if (ns >= 1)
+      a.items[pos] = BinItem{id0, draw, (uint16_t)s0.bx0, (uint16_t)s0.bx1,
+                             (uint16_t)s0.by0, (uint16_t)s0.by1};
+    if (ns == 2)
+      a.items[pos + 1] = BinItem{id1, draw, (uint16_t)s1.bx0, (uint16_t)s1.bx1,
+                                 (uint16_t)s1.by0, (uint16_t)s1.by1};
     auto count = [&](int key, uint32_t n) -> uint32_t {
-      atomicAdd(&a.tile_count[(size_t)key * kCtrStride + 1], n);
+      atomicAdd(&a.tile_count[(size_t)key * kCtrStride], n);
       return 0u;
     };
-    auto bin_sub = [&](bool act, const TriSetup& t, uint32_t id, int64_t p) {
-      const int tx0 = t.bx0 >> sw, tx1 = t.bx1 >> sw, ty0 = t.by0 >> sh, ty1 = t.by1 >> sh;
-      const bool single = act && tx0 == tx1 && ty0 == ty1;
-      const int key = tbase + ty0 * tiles_x + tx0;
-      const uint32_t slot = wave_rank_add(single, key, a.tile_count);
-      wave_tile_pairs<false>(a, act && !single, id, draw, tbase, tiles_x, tx0, tx1, ty0, ty1,
-                             count);
-      if (!act) return;
-      if (single)
-        a.items[p] = BinItem{id, draw, (uint16_t)(t.bx0 | kSingleTile),
-                             (uint16_t)(slot & 0xFFFFu), (uint16_t)t.by0, (uint16_t)(slot >> 16)};
-      else
-        a.items[p] = BinItem{id, draw, (uint16_t)t.bx0, (uint16_t)t.bx1,
-                             (uint16_t)t.by0, (uint16_t)t.by1};
-    };
-    bin_sub(ns >= 1, s0, id0, pos);
-    if (b2) bin_sub(ns == 2, s1, id1, pos + 1);
+    wave_tile_pairs<false>(a, ns >= 1, id0, draw, tbase, tiles_x, s0.bx0 >> sw,
+                           s0.bx1 >> sw, s0.by0 >> sh, s0.by1 >> sh, count);
+    if (b2)
+      wave_tile_pairs<false>(a, ns == 2, id1, draw, tbase, tiles_x, s1.bx0 >> sw,
+                             s1.bx1 >> sw, s1.by0 >> sh, s1.by1 >> sh, count);
   }
   __shared__ uint32_t sh_clipped;
   if (threadIdx.x == 0) sh_clipped = 0;
@@ -479,9 +440,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
   u64 sum = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    v[i] = (base + i < a.ntiles) ? a.tile_count[(size_t)(base + i) * kCtrStride] +
-                                       a.tile_count[(size_t)(base + i) * kCtrStride + 1]
-                                 : 0u;
+    v[i] = (base + i < a.ntiles) ? a.tile_count[(size_t)(base + i) * kCtrStride] : 0u;
     sum += v[i];
   }
   u64 x = sum;  // wave inclusive scan
@@ -551,8 +510,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
     if (t < a.ntiles) {
       const uint32_t o = run > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)run;
       a.tile_off[t] = o;
-      // multi-tile pairs go after the single-tile items' slots
-      a.tile_cursor[(size_t)t * kCtrStride] = o + a.tile_count[(size_t)t * kCtrStride];
+      a.tile_cursor[(size_t)t * kCtrStride] = o;
       uint32_t slot = kNoSlot;
       if (v[i] > a.frag_size) {
         const uint32_t nparts = (v[i] + a.frag_size - 1) / a.frag_size;
@@ -610,16 +568,8 @@ __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
       BinItem it = first;
       if (j > 0) it = act ? a.items[r * kRegion + j + lane] : BinItem{0, 0, 0, 0, 0, 0};
       const DrawDesc& D = a.draws[act ? it.draw : 0u];
-      const bool single = act && (it.bx0 & kSingleTile);
-      if (single) {  // slot taken in the counting pass
-        const int key = D.tile_base + (it.by0 >> sh) * D.tiles_x + ((it.bx0 & 0x7FFF) >> sw);
-        const uint64_t slot =
-            (uint64_t)a.tile_off[key] + ((uint32_t)it.bx1 | ((uint32_t)it.by1 << 16));
-        if (slot < a.bins_cap) a.bins[slot] = make_uint2(it.id, it.draw);
-        else atomicOr(&a.counters[kBinOverflow], 1ull);
-      }
-      wave_tile_pairs<true>(a, act && !single, it.id, it.draw, D.tile_base, D.tiles_x,
-                            it.bx0 >> sw, it.bx1 >> sw, it.by0 >> sh, it.by1 >> sh, reserve);
+      wave_tile_pairs<true>(a, act, it.id, it.draw, D.tile_base, D.tiles_x, it.bx0 >> sw,
+                            it.bx1 >> sw, it.by0 >> sh, it.by1 >> sh, reserve);
     }
     r = rn;
     cnt = cnt_n;
@@ -678,8 +628,7 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
     part = 0;
   }
   {
-    const uint32_t cnt = a.tile_count[(size_t)tile * kCtrStride] +
-                         a.tile_count[(size_t)tile * kCtrStride + 1];
+    const uint32_t cnt = a.tile_count[(size_t)tile * kCtrStride];
     const uint32_t nparts = cnt <= a.frag_size ? 1u : (cnt + a.frag_size - 1) / a.frag_size;
     if (b >= nparts_all && nparts > 1) return;  // rendered through its parts
     const int j = search_last(a.job_tile_base, 0, a.njobs - 1, tile);
@@ -1014,7 +963,6 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
        i += (int64_t)a.shade_blocks * kShadeBlock) {
     if (i < a.ntiles) {
       a.tile_count[(size_t)i * kCtrStride] = 0;
-      a.tile_count[(size_t)i * kCtrStride + 1] = 0;
       a.tile_done[i] = 0;
     }
     if (i < a.scan_blocks) a.scan_status[i] = 0;
