@@ -157,6 +157,45 @@ MR_HD inline float advance(float s, float d, int n) {
   return ff_add(s, d, n);
 }
 
+// Both edge values of a row after n columns: the two recurrences interleaved
+// and unrolled by 8, so a wavefront spends about one instruction per add on
+// loop control instead of three or four (lanes with different n run the same
+// instruction stream under their execution mask).
+MR_HD inline void advance2(float& e1, float d1, float& e2, float d2, int n) {
+  if (n > 256) {
+    e1 = ff_add(e1, d1, n);
+    e2 = ff_add(e2, d2, n);
+    return;
+  }
+  float a = e1, b = e2;
+  for (int i = n >> 3; i > 0; i--) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      a = a + d1;
+      b = b + d2;
+    }
+  }
+  if (n & 4) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      a = a + d1;
+      b = b + d2;
+    }
+  }
+  if (n & 2) {
+    a = a + d1;
+    b = b + d2;
+    a = a + d1;
+    b = b + d2;
+  }
+  if (n & 1) {
+    a = a + d1;
+    b = b + d2;
+  }
+  e1 = a;
+  e2 = b;
+}
+
 // Row start of the edge walk at the bbox's left column (Renderer.cpp:240-242).
 // x0f = floor(pmin.x) + 0.5f, y = row + 0.5f.
 MR_HD inline void edge_start(float n1x, float n1y, float n2x, float n2y,

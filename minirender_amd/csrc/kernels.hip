@@ -102,10 +102,12 @@ __device__ inline int search_last(const T* begin, int lo, int hi, int64_t x) {
   return lo;
 }
 
-// Draw of flat triangle g in job J: the job's draw table cached in LDS when
-// it fits (ncached > 0), else searched in global memory.
+// Draw of flat triangle g in job J: the job's only draw (ncached < 0), the
+// job's draw table cached in LDS when it fits (ncached > 0), else searched in
+// global memory.
 __device__ inline int lookup_draw(const int64_t* dcache, int ncached,
                                   const BatchArgs& a, const JobDesc& J, int64_t g) {
+  if (ncached < 0) return J.draw_begin;  // the job has one draw
   if (ncached > 0) return J.draw_begin + search_last(dcache, 0, ncached - 1, g);
   return search_last(a.draw_begin, J.draw_begin, J.draw_end - 1, g);
 }
@@ -693,8 +695,7 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
         float e1, e2;
         edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
                    (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
-        e1 = advance(e1, s.n1x, cx0 - s.bx0);
-        e2 = advance(e2, s.n2x, cx0 - s.bx0);
+        advance2(e1, s.n1x, e2, s.n2x, cx0 - s.bx0);
         const int prow = (row - y0) * KS - x0;
         const uint32_t id = setup_id[i];
         for (int c = cx0; c <= cx1; c++) {
@@ -802,8 +803,7 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
              (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
   const int n = c - s.bx0;
-  e1 = advance(e1, s.n1x, n);
-  e2 = advance(e2, s.n2x, n);
+  advance2(e1, s.n1x, e2, s.n2x, n);
   float k[3];
   const float z = depth_of(e1, e2, s.z0, s.z1, s.z2, J.persp != 0, k);
   if (!J.rgb8 && !J.rgb && !(J.normals && J.save_normals)) {
@@ -956,7 +956,18 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
   const int b = (int)xcd_remap(blockIdx.x, a.shade_blocks);
   const int j = __builtin_amdgcn_readfirstlane(search_last(a.job_shade_base, 0, a.njobs - 1, b));
   const JobDesc J = uniform_job(a, j);
-  const int ncached = fill_draw_cache(dcache, a, J, tid, kShadeBlock);
+  const int bxn = (J.W + kShadeW - 1) / kShadeW;
+  const int local = b - J.shade_base;
+  const int byi = local / bxn;
+  const int r = tid / kShadeW;
+  const int c = (local - byi * bxn) * kShadeW + (tid - r * kShadeW), row = byi * kShadeH + r;
+  const bool inside = c < J.W && row < J.H;
+  // The winner load is issued first, so that its latency overlaps the
+  // draw-table fill below.
+  const uint32_t id = inside ? J.win[(size_t)row * J.W + c] : kNoWinner;
+  // A job of one draw needs no draw table (ncached = -1).
+  const bool one_draw = J.draw_end - J.draw_begin == 1;
+  const int ncached = one_draw ? -1 : fill_draw_cache(dcache, a, J, tid, kShadeBlock);
   // The raster and scan are done with the tile counters, part counters and
   // look-back words: leave them zero for the next batch on this scratch.
   for (int64_t i = (int64_t)blockIdx.x * kShadeBlock + tid; i < a.ntiles || i < a.scan_blocks;
@@ -967,15 +978,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
     }
     if (i < a.scan_blocks) a.scan_status[i] = 0;
   }
-  __syncthreads();
+  if (!one_draw) __syncthreads();
   if (blockIdx.x == 0) reduce_stats(a, tid);
-  const int bxn = (J.W + kShadeW - 1) / kShadeW;
-  const int local = b - J.shade_base;
-  const int byi = local / bxn;
-  const int r = tid / kShadeW;
-  const int c = (local - byi * bxn) * kShadeW + (tid - r * kShadeW), row = byi * kShadeH + r;
-  if (c < J.W && row < J.H)
-    shade_pixel(a, J, dcache, ncached, J.win[(size_t)row * J.W + c], c, row);
+  if (inside) shade_pixel(a, J, dcache, ncached, id, c, row);
 }
 
 // ---- launchers -------------------------------------------------------------

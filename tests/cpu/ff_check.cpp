@@ -1,4 +1,4 @@
-// CPU check of mr::ff_add / mr::advance (minirender_amd/csrc/exact.h):
+// CPU check of mr::ff_add / mr::advance / mr::advance2 (minirender_amd/csrc/exact.h):
 // n repetitions of the fp32 recurrence s = s + d (Renderer.cpp:243) must be
 // bit-identical to the serial loop.  Built with g++ -O2 -ffp-contract=off.
 //
@@ -70,6 +70,11 @@ static float rand_step() {
   }
 }
 
+// Equal bit patterns, or both NaN (payloads may differ).
+static bool same_value(float x, float y) {
+  return fbits(x) == fbits(y) || (std::isnan(x) && std::isnan(y));
+}
+
 int main(int argc, char** argv) {
   const long cases = argc > 1 ? atol(argv[1]) : 200000;
   rng_state = argc > 2 ? strtoull(argv[2], nullptr, 10) : 1;
@@ -80,10 +85,15 @@ int main(int argc, char** argv) {
     for (int i = 0; i < n; i++) ref = ref + d;
     const float got = mr::ff_add(s0, d, n);
     const float got2 = mr::advance(s0, d, n);
-    const bool same = fbits(ref) == fbits(got) && fbits(ref) == fbits(got2);
-    const bool both_nan = std::isnan(ref) && std::isnan(got) && std::isnan(got2);
-    if (!same && !both_nan) {
-      printf("mismatch s=%a d=%a n=%d ref=%a ff=%a adv=%a\n", s0, d, n, ref, got, got2);
+    // advance2 on the pair (s0 by d) and (d by s0): two recurrences at once
+    float ref2 = d;
+    for (int i = 0; i < n; i++) ref2 = ref2 + s0;
+    float a2 = s0, b2 = d;
+    mr::advance2(a2, d, b2, s0, n);
+    const bool same = same_value(ref, got) && same_value(ref, got2) && same_value(ref, a2) &&
+                      same_value(ref2, b2);
+    if (!same) {
+      printf("mismatch s=%a d=%a n=%d ref=%a ff=%a adv=%a adv2=%a/%a\n", s0, d, n, ref, got, got2, a2, b2);
       return 1;
     }
   }
