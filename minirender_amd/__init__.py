@@ -117,6 +117,9 @@ class Context:
     def set_frag_size(self, pairs: int) -> None:
         check(self.lib.mr_ctx_set_frag_size(self.h, pairs), "mr_ctx_set_frag_size")
 
+    def set_edge_handoff(self, on: bool) -> None:
+        check(self.lib.mr_ctx_set_edge_handoff(self.h, int(on)), "mr_ctx_set_edge_handoff")
+
     def set_bin_factor(self, f: float) -> None:
         check(self.lib.mr_ctx_set_bin_factor(self.h, f), "mr_ctx_set_bin_factor")
 
@@ -231,7 +234,8 @@ class Context:
               "mr_ctx_autotune")
         return {"tile": t.tile, "raster_threads": t.raster_threads,
                 "bin_factor": round(t.bin_factor, 3), "frag_size": t.frag_size,
-                "ms": t.ms, "from_cache": bool(t.from_cache), "candidates": t.candidates}
+                "edge_handoff": bool(t.edge_handoff), "ms": t.ms,
+                "from_cache": bool(t.from_cache), "candidates": t.candidates}
 
 
 class Server:

@@ -586,27 +586,55 @@ __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
     a.block_stats[kStatSlots + blockIdx.x] = sh_items;
 }
 
-// Depth test of one pixel into the tile's LDS key buffer.
-__device__ inline void raster_pixel(u64* keys, int p, float e1, float e2,
+// Edge values of a pixel not (yet) handed to shading: a quiet NaN.
+constexpr uint32_t kEdgeUnset = 0x7FC00000u;
+
+// Items whose walk from the bbox's left column to their last column in the
+// tile is shorter than this leave the edge values to the shading kernel
+// (a short serial walk there); longer ones hand them over.
+constexpr int kEdgeHandoff = 16;
+
+// A (triangle, row) item that hands its edge values over: its setup slot,
+// row and edge values at its first column in the tile.
+struct Handoff {
+  int slot, row;
+  float e1, e2;
+};
+
+// Depth test of one pixel into the tile's LDS key buffer.  A lane that may
+// change the pixel's winner marks its handed-over edge values unset (every
+// such writer stores the same value, so the stores need no ordering).
+template <bool HO>
+__device__ inline void raster_pixel(u64* keys, float2* ebuf, int p, float e1, float e2,
                                     const TriSetup& s, bool persp, uint32_t id) {
   if (!covered(e1, e2)) return;
   float k[3];
   const float z = depth_of(e1, e2, s.z0, s.z1, s.z2, persp, k);
   if (!(z < kClearDepth)) return;  // strict test against the clear value
   const u64 key = ((u64)depth_key(z) << 32) | id;
-  if (key < keys[p]) atomicMin(&keys[p], key);
+  if (key < keys[p]) {
+    atomicMin(&keys[p], key);
+    if (HO) reinterpret_cast<uint32_t*>(ebuf)[2 * p] = kEdgeUnset;
+  }
 }
 
 // (d): one workgroup per raster work item: first the parts of split tiles
 // (listed by the scan), then every tile.  The grid has parts_cap + ntiles
 // workgroups; workgroups past the actual number of items exit at once.
-template <int TW, int TH, int NT>
+template <int TW, int TH, int NT, bool HO>
 __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
   constexpr int NP = TW * TH;
   // Key rows padded by one entry: pixels of one column in different rows
   // fall in different LDS banks.
   constexpr int KS = TW + 1;
+  // With the edge hand-over (HO): the winners' edge values per pixel, and
+  // the items that hand theirs over, listed by the raster (at most QCAP per
+  // round; further ones leave the walk to shading).
+  constexpr int QCAP = HO ? NT : 1;
   __shared__ u64 keys[TH * KS];
+  __shared__ float2 ebuf[HO ? TH * KS : 1];
+  __shared__ Handoff hlist[QCAP];
+  __shared__ int hcount;
   __shared__ TriSetup setups[NT];
   __shared__ uint32_t setup_id[NT];
   __shared__ int row_start[NT];
@@ -640,6 +668,7 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
     const int tyi = local / J.tiles_x, txi = local - tyi * J.tiles_x;
     const int x0 = txi * TW, y0 = tyi * TH;
     for (int p = tid; p < TH * KS; p += NT) keys[p] = ~0ull;
+    if (tid == 0) hcount = 0;
     const uint32_t off = a.tile_off[tile];
     uint32_t avail = cnt;  // entries actually written (bin-list overflow)
     if ((u64)off >= a.bins_cap) avail = 0;
@@ -696,21 +725,52 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
         edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
                    (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
         advance2(e1, s.n1x, e2, s.n2x, cx0 - s.bx0);
+        if (HO && cx1 - s.bx0 >= kEdgeHandoff) {
+          const int h = atomicAdd(&hcount, 1);
+          if (h < QCAP) hlist[h] = Handoff{i, row, e1, e2};
+        }
         const int prow = (row - y0) * KS - x0;
         const uint32_t id = setup_id[i];
         for (int c = cx0; c <= cx1; c++) {
-          raster_pixel(keys, prow + c, e1, e2, s, persp, id);
+          raster_pixel<HO>(keys, ebuf, prow + c, e1, e2, s, persp, id);
           e1 = e1 + s.n1x;
           e2 = e2 + s.n2x;
         }
+      }
+      __syncthreads();
+      // Edge hand-over: each listed item walks its row again and stores its
+      // edge values at the pixels it currently wins.  A pixel has one winner,
+      // so every pixel has at most one writer; a later round that takes the
+      // pixel over unsets the values in its raster pass first.
+      const int nh = HO ? min(hcount, QCAP) : 0;
+      for (int h = tid; h < nh; h += NT) {
+        const Handoff hd = hlist[h];
+        const TriSetup& s = setups[hd.slot];
+        const int cx0 = max(s.bx0, x0), cx1 = min(s.bx1, x0 + TW - 1);
+        const int prow = (hd.row - y0) * KS - x0;
+        const uint32_t id = setup_id[hd.slot];
+        const uint32_t* key_lo = reinterpret_cast<const uint32_t*>(keys);
+        float e1 = hd.e1, e2 = hd.e2;
+        for (int c = cx0; c <= cx1; c++) {
+          if (key_lo[2 * (prow + c)] == id) ebuf[prow + c] = make_float2(e1, e2);
+          e1 = e1 + s.n1x;
+          e2 = e2 + s.n2x;
+        }
+      }
+      if (HO) {
+        __syncthreads();
+        if (tid == 0) hcount = 0;
       }
       __syncthreads();
     }
     if (nparts == 1) {
       for (int p = tid; p < NP; p += NT) {
         const int c = x0 + (p % TW), row = y0 + (p / TW), lp = (p / TW) * KS + (p % TW);
-        if (c < J.W && row < J.H)
-          J.win[(size_t)row * J.W + c] = keys[lp] == ~0ull ? kNoWinner : (uint32_t)keys[lp];
+        if (c < J.W && row < J.H) {
+          const size_t pix = (size_t)row * J.W + c;
+          J.win[pix] = keys[lp] == ~0ull ? kNoWinner : (uint32_t)keys[lp];
+          if (HO) reinterpret_cast<float2*>(J.edge)[pix] = ebuf[lp];
+        }
       }
     } else {
       // Split tile: merge into the tile's global key buffer; the last part
@@ -739,8 +799,12 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
           for (int p = tid; p < NP; p += NT) {
             const int c = x0 + (p % TW), row = y0 + (p / TW);
             const u64 k = __hip_atomic_load(&gk[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (c < J.W && row < J.H)
-              J.win[(size_t)row * J.W + c] = k == ~0ull ? kNoWinner : (uint32_t)k;
+            if (c < J.W && row < J.H) {
+              // the parts' edge values are not merged: shading walks again
+              const size_t pix = (size_t)row * J.W + c;
+              J.win[pix] = k == ~0ull ? kNoWinner : (uint32_t)k;
+              if (HO) reinterpret_cast<uint2*>(J.edge)[pix] = make_uint2(kEdgeUnset, kEdgeUnset);
+            }
           }
         }
       }
@@ -763,7 +827,7 @@ __device__ inline f3 fast_norm3(f3 a) { return scale3(fast_rlen3(a), a); }
 
 // Shading of a covered pixel whose winner `id` belongs to draw D.
 __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, const DrawDesc& D,
-                                          uint32_t id, int c, int row, size_t pix) {
+                                          uint32_t id, float2 ev, int c, int row, size_t pix) {
   const int64_t g = id >> 1;
   const int sub = (int)(id & 1u);
   const int tri = (int)(g - D.tri_begin);
@@ -799,11 +863,14 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   v2.p = ec[2];
   TriSetup s;
   tri_setup_corners(J, ec, vc, sub, s);
-  float e1, e2;
-  edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
-             (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
-  const int n = c - s.bx0;
-  advance2(e1, s.n1x, e2, s.n2x, n);
+  // Edge values at the pixel: handed over by the raster (the same serial
+  // walk), else walked here from the bbox's left column.
+  float e1 = ev.x, e2 = ev.y;
+  if (!(e1 == e1)) {
+    edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
+               (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
+    advance2(e1, s.n1x, e2, s.n2x, c - s.bx0);
+  }
   float k[3];
   const float z = depth_of(e1, e2, s.z0, s.z1, s.z2, J.persp != 0, k);
   if (!J.rgb8 && !J.rgb && !(J.normals && J.save_normals)) {
@@ -883,7 +950,7 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
 
 // Deferred shading of one pixel from its winner (Renderer.cpp:271-305).
 __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
-                            const int64_t* dcache, int ncached, uint32_t id,
+                            const int64_t* dcache, int ncached, uint32_t id, float2 ev,
                             int c, int row) {
   const size_t pix = (size_t)row * (size_t)J.W + (size_t)c;
   if (id == kNoWinner) {  // clear(), Renderer.cpp:113-119
@@ -912,9 +979,9 @@ __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
   if (__ballot(d == d0) == __ballot(1)) {
     // One draw for the whole wavefront: its descriptor through scalar loads.
     const DrawDesc Du = uniform_draw(a, d0);
-    shade_lit(a, J, Du, id, c, row, pix);
+    shade_lit(a, J, Du, id, ev, c, row, pix);
   } else {
-    shade_lit(a, J, a.draws[d], id, c, row, pix);
+    shade_lit(a, J, a.draws[d], id, ev, c, row, pix);
   }
 }
 
@@ -962,9 +1029,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
   const int r = tid / kShadeW;
   const int c = (local - byi * bxn) * kShadeW + (tid - r * kShadeW), row = byi * kShadeH + r;
   const bool inside = c < J.W && row < J.H;
-  // The winner load is issued first, so that its latency overlaps the
-  // draw-table fill below.
-  const uint32_t id = inside ? J.win[(size_t)row * J.W + c] : kNoWinner;
+  // The winner and edge loads are issued first, so that their latency
+  // overlaps the draw-table fill below.
+  const size_t pix = (size_t)row * J.W + c;
+  const uint32_t id = inside ? J.win[pix] : kNoWinner;
+  const float2 ev = inside && a.edge_handoff
+                        ? reinterpret_cast<const float2*>(J.edge)[pix]
+                        : make_float2(__uint_as_float(kEdgeUnset), 0.0f);
   // A job of one draw needs no draw table (ncached = -1).
   const bool one_draw = J.draw_end - J.draw_begin == 1;
   const int ncached = one_draw ? -1 : fill_draw_cache(dcache, a, J, tid, kShadeBlock);
@@ -980,7 +1051,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
   }
   if (!one_draw) __syncthreads();
   if (blockIdx.x == 0) reduce_stats(a, tid);
-  if (inside) shade_pixel(a, J, dcache, ncached, id, c, row);
+  if (inside) shade_pixel(a, J, dcache, ncached, id, ev, c, row);
 }
 
 // ---- launchers -------------------------------------------------------------
@@ -1033,7 +1104,10 @@ bool raster_threads_supported(int tw, int nt) {
 
 #define MR_TILE_CASE(TW, NT)                                                         \
   if (a.tile_w == TW && a.raster_threads == NT) {                                     \
-    hipLaunchKernelGGL((k_tile<TW, TW, NT>), dim3(grid), dim3(NT), 0, s, a);          \
+    if (a.edge_handoff)                                                               \
+      hipLaunchKernelGGL((k_tile<TW, TW, NT, true>), dim3(grid), dim3(NT), 0, s, a);  \
+    else                                                                              \
+      hipLaunchKernelGGL((k_tile<TW, TW, NT, false>), dim3(grid), dim3(NT), 0, s, a); \
     return hipGetLastError();                                                         \
   }
 
