@@ -112,6 +112,11 @@ MR_HD inline int64_t xcd_remap(int64_t b, int64_t n) {
 // binner then spread over all memory channels instead of a few.
 constexpr int kCtrStride = 16;
 
+// The edge-checkpoint buffer is cut into this many segments, each with its
+// own allocation counter (kCtrStride words apart): one shared counter would
+// serialize every binning wavefront's atomic.
+constexpr int kCkptSegs = 64;
+
 constexpr int kRegion = 128;       // item slots per geometry wavefront (64 x 2)
 constexpr int kShadeBlock = 256;   // pixels per deferred-shading workgroup
 constexpr int kShadeW = 64;        // shading block: kShadeW x kShadeH pixels

@@ -110,7 +110,7 @@ void slot_free(mr_ctx* c, Slot& s) {
   Scratch& x = s.sc;
   DevBuf* bufs[] = {&x.items, &x.region_count, &x.tile_count, &x.tile_off,
                     &x.tile_cursor, &x.tile_heavy, &x.tile_done, &x.parts, &x.bins,
-                    &x.heavy_keys, &x.scan_status, &x.counters, &x.block_stats,
+                    &x.ckpt_base, &x.ckpt, &x.ckpt_ctr, &x.heavy_keys, &x.scan_status, &x.counters, &x.block_stats,
                     &x.tables, &x.vrec, &x.win, &x.outputs};
   for (DevBuf* b : bufs) dev_free(c, *b, true);
   HostBuf* hbufs[] = {&x.h_tables, &x.h_counters, &x.h_outputs};
@@ -252,6 +252,10 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
   return MR_OK;
 }
 
+// Edge checkpoints of a batch: two per pixel covers every frame here with
+// room; a batch that needs more leaves the extra rows to shading's own walk.
+static size_t ckpt_entries(int64_t npixels) { return (size_t)npixels * 2 + 65536; }
+
 // Per-pixel scratch of a batch: winner ids (4 B), then from the next 256-byte
 // boundary the winners' edge values (8 B).
 static size_t win_bytes(int64_t npixels) { return ((size_t)npixels * 4 + 255) & ~(size_t)255; }
@@ -275,6 +279,9 @@ static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins
   if ((rc = dev_alloc(c, s.tile_done, nt, true))) return rc;
   if ((rc = dev_alloc(c, s.parts, parts * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.bins, (size_t)bins_cap * 8, true))) return rc;
+  if ((rc = dev_alloc(c, s.ckpt_base, (size_t)std::max<int64_t>(2 * T, 1) * 4, true))) return rc;
+  if ((rc = dev_alloc(c, s.ckpt, ckpt_entries(p.npixels) * 8, true))) return rc;
+  if ((rc = dev_alloc(c, s.ckpt_ctr, kCkptSegs * kCtrStride * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.heavy_keys, (size_t)heavy_slots * npt, true))) return rc;
   if ((rc = dev_alloc(c, s.scan_status, (size_t)std::max(scb, 1) * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.counters, kNumCounters * 8, true))) return rc;
@@ -304,6 +311,10 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.bins = (uint2*)s.bins.p;
   a.bins_cap = s.bins.bytes / 8;
   if (a.bins_cap > 0xFFFFFFF0ull) a.bins_cap = 0xFFFFFFF0ull;
+  a.ckpt_base = (uint32_t*)s.ckpt_base.p;
+  a.ckpt = (float2*)s.ckpt.p;
+  a.ckpt_cap = std::min<uint64_t>(s.ckpt.bytes / 8, 0xFFFFFFF0ull);
+  a.ckpt_ctr = (unsigned long long*)s.ckpt_ctr.p;
   a.heavy_keys = (unsigned long long*)s.heavy_keys.p;
   a.heavy_cap = (uint32_t)(s.heavy_keys.bytes / ((size_t)c->tile_w * c->tile_h * 8));
   a.frag_size = c->frag_size;

@@ -44,8 +44,8 @@ struct TexRec {
 // Device scratch of one batch in flight, and its pinned staging.
 struct Scratch {
   DevBuf items, region_count, tile_count, tile_off, tile_cursor, tile_heavy,
-      tile_done, parts, bins, heavy_keys, scan_status, counters, block_stats,
-      tables, vrec, win, outputs;
+      tile_done, parts, bins, ckpt_base, ckpt, ckpt_ctr, heavy_keys, scan_status, counters,
+      block_stats, tables, vrec, win, outputs;
   // The batch tables (jobs, job tile/shade bases, draws, draw starts) sit in
   // one pinned staging buffer and one device buffer, uploaded by one copy.
   HostBuf h_tables, h_counters, h_outputs;

@@ -345,8 +345,10 @@ __device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id, ui
 // (a)+(b): one lane per input triangle; wavefront w writes its survivors to
 // items[w * kRegion ...] and their number to region_count[w].
 __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
-  // The batch counters are first used by the scan: clear them here.
+  // The batch counters and checkpoint segment counters are first used by the
+  // scan and the binner: clear them here.
   if (blockIdx.x == 0 && threadIdx.x < kNumCounters) a.counters[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < kCkptSegs) a.ckpt_ctr[threadIdx.x * kCtrStride] = 0;
   const int lane = threadIdx.x & 63;
   const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
   uint32_t nclipped = 0;
@@ -531,8 +533,38 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
   }
 }
 
-// (c) scatter triangle ids into the bin lists of the tiles they overlap;
-// also resets the merge buffers of split tiles.
+constexpr uint32_t kNoCkpt = 0xFFFFFFFFu;
+
+// Edge checkpoints of the wavefront's items: an item spanning ntx > 0 tile
+// columns after its first gets ntx entries per pixel row (one per column it
+// enters), taken by one atomic per wavefront from the checkpoint segment of
+// the wavefront (wavefronts spread over kCkptSegs segments).  Items without
+// checkpoints (one tile column, or no room left) get kNoCkpt.
+__device__ inline void ckpt_alloc(const BatchArgs& a, bool act, const BinItem& it, int sw) {
+  const int lane = threadIdx.x & 63;
+  const int ntx = act ? (it.bx1 >> sw) - (it.bx0 >> sw) : 0;
+  const u64 need = ntx > 0 ? (u64)ntx * (u64)(it.by1 - it.by0 + 1) : 0ull;
+  u64 P = need;  // inclusive prefix over the wave
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const u64 y = __shfl_up(P, o);
+    if (lane >= o) P += y;
+  }
+  const u64 total = __shfl(P, 63);
+  const u64 seg_cap = a.ckpt_cap / kCkptSegs;
+  const int seg = (int)((blockIdx.x * 4u + (threadIdx.x >> 6)) % kCkptSegs);
+  u64 base = 0;
+  if (total && lane == 63) base = atomicAdd(&a.ckpt_ctr[seg * kCtrStride], total);
+  base = __shfl(base, 63);
+  if (act) {
+    const u64 first = base + P - need;  // within the segment
+    a.ckpt_base[it.id] = (need && first + need <= seg_cap)
+                             ? (uint32_t)(seg * seg_cap + first) : kNoCkpt;
+  }
+}
+
+// (c) scatter triangle ids into the bin lists of the tiles they overlap and
+// take their edge checkpoints; also resets the merge buffers of split tiles.
 __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
   const int lane = threadIdx.x & 63;
   const u64 np = (u64)a.tile_w * a.tile_h;
@@ -570,6 +602,7 @@ __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
       BinItem it = first;
       if (j > 0) it = act ? a.items[r * kRegion + j + lane] : BinItem{0, 0, 0, 0, 0, 0};
       const DrawDesc& D = a.draws[act ? it.draw : 0u];
+      ckpt_alloc(a, act, it, sw);
       wave_tile_pairs<true>(a, act, it.id, it.draw, D.tile_base, D.tiles_x, it.bx0 >> sw,
                             it.bx1 >> sw, it.by0 >> sh, it.by1 >> sh, reserve);
     }
@@ -725,12 +758,20 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
         edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
                    (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
         advance2(e1, s.n1x, e2, s.n2x, cx0 - s.bx0);
+        const uint32_t id = setup_id[i];
+        if (cx0 > s.bx0) {  // entering a tile column after the first: checkpoint
+          const uint32_t cb = a.ckpt_base[id];
+          if (cb != kNoCkpt) {
+            const int btx = s.bx0 / TW, ntx = s.bx1 / TW - btx;
+            a.ckpt[cb + (uint32_t)((row - s.by0) * ntx + (x0 / TW - btx - 1))] =
+                make_float2(e1, e2);
+          }
+        }
         if (HO && cx1 - s.bx0 >= kEdgeHandoff) {
           const int h = atomicAdd(&hcount, 1);
           if (h < QCAP) hlist[h] = Handoff{i, row, e1, e2};
         }
         const int prow = (row - y0) * KS - x0;
-        const uint32_t id = setup_id[i];
         for (int c = cx0; c <= cx1; c++) {
           raster_pixel<HO>(keys, ebuf, prow + c, e1, e2, s, persp, id);
           e1 = e1 + s.n1x;
@@ -835,6 +876,7 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   // of memory access instead of four.
   const int* ip = D.idx + 3 * tri;
   const int* in = D.nidx + 3 * tri;
+  const uint32_t cb = a.ckpt_base[id];
   const int i0 = ip[0], i1 = ip[1], i2 = ip[2];
   const int n0 = in[0], n1 = in[1], n2 = in[2];
   int t0 = 0, t1 = 0, t2 = 0;
@@ -863,13 +905,24 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   v2.p = ec[2];
   TriSetup s;
   tri_setup_corners(J, ec, vc, sub, s);
-  // Edge values at the pixel: handed over by the raster (the same serial
-  // walk), else walked here from the bbox's left column.
+  // Edge values at the pixel, all from the same serial walk along the row:
+  // handed over by the raster, or walked from the raster's checkpoint where
+  // the row enters the pixel's tile column, or from the bbox's left column.
   float e1 = ev.x, e2 = ev.y;
   if (!(e1 == e1)) {
-    edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
-               (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
-    advance2(e1, s.n1x, e2, s.n2x, c - s.bx0);
+    const int sw = __ffs(a.tile_w) - 1;
+    const int btx = s.bx0 >> sw, ctx = c >> sw;
+    if (cb != kNoCkpt && ctx > btx) {
+      const float2 st =
+          a.ckpt[cb + (uint32_t)((row - s.by0) * ((s.bx1 >> sw) - btx) + (ctx - btx - 1))];
+      e1 = st.x;
+      e2 = st.y;
+      advance2(e1, s.n1x, e2, s.n2x, c - (ctx << sw));
+    } else {
+      edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
+                 (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
+      advance2(e1, s.n1x, e2, s.n2x, c - s.bx0);
+    }
   }
   float k[3];
   const float z = depth_of(e1, e2, s.z0, s.z1, s.z2, J.persp != 0, k);

@@ -21,6 +21,12 @@ struct BatchArgs {
   uint32_t parts_cap;
   uint2* bins;               // [bins_cap] (triangle id, draw) grouped by tile
   uint64_t bins_cap;
+  // Edge checkpoints: the edge values of a triangle row where it enters each
+  // tile column after its first (written by the raster, read by shading).
+  uint32_t* ckpt_base;       // [2 * total_tris] first checkpoint of a survivor id
+  float2* ckpt;              // [ckpt_cap], kCkptSegs equal segments
+  uint64_t ckpt_cap;
+  unsigned long long* ckpt_ctr;  // [kCkptSegs * kCtrStride] segment fill counters
   unsigned long long* heavy_keys;  // [heavy_cap * tile pixels] merge buffers
   uint32_t heavy_cap;
   uint32_t frag_size;        // max (tile, triangle) pairs per raster work item

@@ -61,7 +61,7 @@ struct Flight {
 // regions; tile counters 2 x 64 B + 16 B), a conservative estimate.
 uint64_t request_bytes(mr_ctx* c, const mr_frame& f, const mr_outputs& o) {
   const uint64_t npix = (uint64_t)f.width * f.height;
-  uint64_t b = npix * 12;  // winner ids + edge values
+  uint64_t b = npix * 28;  // winner ids, edge values, edge checkpoints
   if (o.rgb8) b += npix * 3;
   if (o.depth) b += npix * 4;
   if (o.winner) b += npix * 4;
@@ -75,7 +75,7 @@ uint64_t request_bytes(mr_ctx* c, const mr_frame& f, const mr_outputs& o) {
   }
   const uint64_t tiles = ((uint64_t)(f.width + c->tile_w - 1) / c->tile_w) *
                          ((uint64_t)(f.height + c->tile_h - 1) / c->tile_h);
-  return b + tris * 48 + tiles * 144;
+  return b + tris * 56 + tiles * 144;  // + 8 B per triangle: checkpoint bases
 }
 
 }  // namespace
