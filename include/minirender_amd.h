@@ -139,7 +139,6 @@ typedef struct mr_tuning {
   double ms;               /* median time of one call with this choice         */
   int32_t from_cache;      /* 1 if read from cache_path                        */
   int32_t candidates;      /* configurations timed                             */
-  int32_t edge_handoff;    /* raster hands winners' edge values to shading     */
 } mr_tuning;
 int mr_ctx_autotune(mr_ctx* ctx, const mr_frame* frames, int32_t nframes,
                     const struct mr_outputs* outs, int32_t reps,
@@ -148,12 +147,6 @@ int mr_ctx_autotune(mr_ctx* ctx, const mr_frame* frames, int32_t nframes,
  * tile with more is split into parts merged through device memory
  * (64 .. 1<<20, default 1024). */
 int mr_ctx_set_frag_size(mr_ctx* ctx, int32_t pairs);
-/* Let the raster hand each pixel's winning edge values (its exact walk
- * values) to shading, which then skips its own walk along the row.  Costs
- * raster LDS and a second pass over long rows; pays off for large
- * triangles.  Off by default; the autotuner tries both.  No counterpart in
- * the reference (its single pass shades while walking). */
-int mr_ctx_set_edge_handoff(mr_ctx* ctx, int32_t on);
 /* Bin-list capacity in (tile, triangle) entries per input triangle
  * (>= 1); overflow re-runs the frame with a larger list. */
 int mr_ctx_set_bin_factor(mr_ctx* ctx, float entries_per_triangle);

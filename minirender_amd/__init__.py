@@ -117,9 +117,6 @@ class Context:
     def set_frag_size(self, pairs: int) -> None:
         check(self.lib.mr_ctx_set_frag_size(self.h, pairs), "mr_ctx_set_frag_size")
 
-    def set_edge_handoff(self, on: bool) -> None:
-        check(self.lib.mr_ctx_set_edge_handoff(self.h, int(on)), "mr_ctx_set_edge_handoff")
-
     def set_bin_factor(self, f: float) -> None:
         check(self.lib.mr_ctx_set_bin_factor(self.h, f), "mr_ctx_set_bin_factor")
 
@@ -234,8 +231,7 @@ class Context:
               "mr_ctx_autotune")
         return {"tile": t.tile, "raster_threads": t.raster_threads,
                 "bin_factor": round(t.bin_factor, 3), "frag_size": t.frag_size,
-                "edge_handoff": bool(t.edge_handoff), "ms": t.ms,
-                "from_cache": bool(t.from_cache), "candidates": t.candidates}
+                "ms": t.ms, "from_cache": bool(t.from_cache), "candidates": t.candidates}
 
 
 class Server:

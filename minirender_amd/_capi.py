@@ -68,7 +68,7 @@ class mr_tuning(C.Structure):
     _fields_ = [
         ("tile", C.c_int32), ("raster_threads", C.c_int32), ("bin_factor", C.c_float),
         ("frag_size", C.c_int32), ("ms", C.c_double), ("from_cache", C.c_int32),
-        ("candidates", C.c_int32), ("edge_handoff", C.c_int32),
+        ("candidates", C.c_int32),
     ]
 
 
@@ -104,7 +104,6 @@ SIGNATURES = {
     "mr_ctx_set_bin_factor": (C.c_int, [_P, _F]),
     "mr_ctx_set_raster_threads": (C.c_int, [_P, _I]),
     "mr_ctx_set_frag_size": (C.c_int, [_P, _I]),
-    "mr_ctx_set_edge_handoff": (C.c_int, [_P, _I]),
     "mr_ctx_set_profiling": (C.c_int, [_P, _I]),
     "mr_ctx_set_graphs": (C.c_int, [_P, _I]),
     "mr_ctx_get_stats": (C.c_int, [_P, C.POINTER(mr_stats)]),

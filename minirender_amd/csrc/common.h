@@ -59,8 +59,6 @@ struct JobDesc {
   int32_t save_normals;
   int32_t shade_base;  // first shading block of this job
   uint32_t* win;       // device scratch: draw-order id of each pixel's winner
-  float* edge;         // device scratch, 2 per pixel: the winner's edge values
-                       // at the pixel from the raster's walk, or NaN (recompute)
   uint8_t* rgb8;       // device outputs (any may be null)
   float* depth;
   uint32_t* winner;

@@ -256,10 +256,6 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
 // room; a batch that needs more leaves the extra rows to shading's own walk.
 static size_t ckpt_entries(int64_t npixels) { return (size_t)npixels * 2 + 65536; }
 
-// Per-pixel scratch of a batch: winner ids (4 B), then from the next 256-byte
-// boundary the winners' edge values (8 B).
-static size_t win_bytes(int64_t npixels) { return ((size_t)npixels * 4 + 255) & ~(size_t)255; }
-
 static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins_cap,
                           uint32_t heavy_slots) {
   Scratch& s = sl.sc;
@@ -288,7 +284,7 @@ static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins
   if ((rc = dev_alloc(c, s.block_stats, 2 * kStatSlots * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.tables, p.layout.bytes, true))) return rc;
   if ((rc = dev_alloc(c, s.vrec, (size_t)p.total_verts * kVrecFloats * 4, true))) return rc;
-  if ((rc = dev_alloc(c, s.win, win_bytes(p.npixels) + (size_t)p.npixels * 8, true))) return rc;
+  if ((rc = dev_alloc(c, s.win, (size_t)p.npixels * 4, true))) return rc;
   if ((rc = host_alloc(s.h_tables, p.layout.bytes))) return rc;
   if ((rc = host_alloc(s.h_counters, kNumCounters * 8))) return rc;
   return MR_OK;
@@ -339,7 +335,6 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.tile_w = c->tile_w;
   a.tile_h = c->tile_h;
   a.raster_threads = c->raster_threads;
-  a.edge_handoff = c->edge_handoff ? 1 : 0;
   a.scan_blocks = scan_blocks_for((int32_t)p.ntiles);
   a.shade_blocks = (int32_t)p.shade_blocks;
   a.num_cus = c->num_cus;
@@ -379,12 +374,9 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   if (rc) return rc;
   Scratch& x = s.sc;
   uint32_t* win = (uint32_t*)x.win.p;
-  float* edge = (float*)((char*)x.win.p + win_bytes(p.npixels));
   for (auto& J : p.jobs) {
     J.win = win;
-    J.edge = edge;
     win += (size_t)J.W * J.H;
-    edge += (size_t)J.W * J.H * 2;
   }
   // Host tables go to pinned staging first; the uploads read them from there
   // (also on a graph replay, which re-reads the same staging).
@@ -687,13 +679,6 @@ int mr_ctx_set_raster_threads(mr_ctx* c, int32_t threads) {
   if (threads == 0) threads = default_raster_threads(c->tile_w);
   if (!raster_threads_supported(c->tile_w, threads)) return MR_ERR_ARG;
   c->raster_threads = threads;
-  return MR_OK;
-}
-
-int mr_ctx_set_edge_handoff(mr_ctx* c, int32_t on) {
-  if (!c) return MR_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(c->mu);
-  c->edge_handoff = on != 0;
   return MR_OK;
 }
 

@@ -113,7 +113,6 @@ struct mr_ctx {
   int32_t raster_threads = 256;  // tile raster workgroup size
   float bin_factor = 3.0f;
   uint32_t frag_size = 1024;  // max (tile, triangle) pairs per raster item
-  bool edge_handoff = false;  // raster hands winners' edge values to shading
   bool profiling = false;
   bool graphs = false;  // capture steady-state batches as HIP graphs
   mr_stats stats{};

@@ -619,55 +619,27 @@ __global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
     a.block_stats[kStatSlots + blockIdx.x] = sh_items;
 }
 
-// Edge values of a pixel not (yet) handed to shading: a quiet NaN.
-constexpr uint32_t kEdgeUnset = 0x7FC00000u;
-
-// Items whose walk from the bbox's left column to their last column in the
-// tile is shorter than this leave the edge values to the shading kernel
-// (a short serial walk there); longer ones hand them over.
-constexpr int kEdgeHandoff = 16;
-
-// A (triangle, row) item that hands its edge values over: its setup slot,
-// row and edge values at its first column in the tile.
-struct Handoff {
-  int slot, row;
-  float e1, e2;
-};
-
-// Depth test of one pixel into the tile's LDS key buffer.  A lane that may
-// change the pixel's winner marks its handed-over edge values unset (every
-// such writer stores the same value, so the stores need no ordering).
-template <bool HO>
-__device__ inline void raster_pixel(u64* keys, float2* ebuf, int p, float e1, float e2,
+// Depth test of one pixel into the tile's LDS key buffer.
+__device__ inline void raster_pixel(u64* keys, int p, float e1, float e2,
                                     const TriSetup& s, bool persp, uint32_t id) {
   if (!covered(e1, e2)) return;
   float k[3];
   const float z = depth_of(e1, e2, s.z0, s.z1, s.z2, persp, k);
   if (!(z < kClearDepth)) return;  // strict test against the clear value
   const u64 key = ((u64)depth_key(z) << 32) | id;
-  if (key < keys[p]) {
-    atomicMin(&keys[p], key);
-    if (HO) reinterpret_cast<uint32_t*>(ebuf)[2 * p] = kEdgeUnset;
-  }
+  if (key < keys[p]) atomicMin(&keys[p], key);
 }
 
 // (d): one workgroup per raster work item: first the parts of split tiles
 // (listed by the scan), then every tile.  The grid has parts_cap + ntiles
 // workgroups; workgroups past the actual number of items exit at once.
-template <int TW, int TH, int NT, bool HO>
-__global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
+template <int TW, int TH, int NT>
+__global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
   constexpr int NP = TW * TH;
   // Key rows padded by one entry: pixels of one column in different rows
   // fall in different LDS banks.
   constexpr int KS = TW + 1;
-  // With the edge hand-over (HO): the winners' edge values per pixel, and
-  // the items that hand theirs over, listed by the raster (at most QCAP per
-  // round; further ones leave the walk to shading).
-  constexpr int QCAP = HO ? NT : 1;
   __shared__ u64 keys[TH * KS];
-  __shared__ float2 ebuf[HO ? TH * KS : 1];
-  __shared__ Handoff hlist[QCAP];
-  __shared__ int hcount;
   __shared__ TriSetup setups[NT];
   __shared__ uint32_t setup_id[NT];
   __shared__ int row_start[NT];
@@ -701,7 +673,6 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
     const int tyi = local / J.tiles_x, txi = local - tyi * J.tiles_x;
     const int x0 = txi * TW, y0 = tyi * TH;
     for (int p = tid; p < TH * KS; p += NT) keys[p] = ~0ull;
-    if (tid == 0) hcount = 0;
     const uint32_t off = a.tile_off[tile];
     uint32_t avail = cnt;  // entries actually written (bin-list overflow)
     if ((u64)off >= a.bins_cap) avail = 0;
@@ -754,53 +725,26 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
         const TriSetup& s = setups[i];
         const int row = max(s.by0, y0) + (q - row_start[i]);
         const int cx0 = max(s.bx0, x0), cx1 = min(s.bx1, x0 + TW - 1);
+        const uint32_t id = setup_id[i];
+        // A row entering a tile column after its first leaves a checkpoint
+        // there; the base is loaded first, so its latency overlaps the walk.
+        uint32_t cb = kNoCkpt, cslot = 0;
+        if (cx0 > s.bx0) {
+          cb = a.ckpt_base[id];
+          const int btx = s.bx0 / TW;
+          cslot = (uint32_t)((row - s.by0) * (s.bx1 / TW - btx) + (x0 / TW - btx - 1));
+        }
         float e1, e2;
         edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
                    (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
         advance2(e1, s.n1x, e2, s.n2x, cx0 - s.bx0);
-        const uint32_t id = setup_id[i];
-        if (cx0 > s.bx0) {  // entering a tile column after the first: checkpoint
-          const uint32_t cb = a.ckpt_base[id];
-          if (cb != kNoCkpt) {
-            const int btx = s.bx0 / TW, ntx = s.bx1 / TW - btx;
-            a.ckpt[cb + (uint32_t)((row - s.by0) * ntx + (x0 / TW - btx - 1))] =
-                make_float2(e1, e2);
-          }
-        }
-        if (HO && cx1 - s.bx0 >= kEdgeHandoff) {
-          const int h = atomicAdd(&hcount, 1);
-          if (h < QCAP) hlist[h] = Handoff{i, row, e1, e2};
-        }
+        if (cb != kNoCkpt) a.ckpt[cb + cslot] = make_float2(e1, e2);
         const int prow = (row - y0) * KS - x0;
         for (int c = cx0; c <= cx1; c++) {
-          raster_pixel<HO>(keys, ebuf, prow + c, e1, e2, s, persp, id);
+          raster_pixel(keys, prow + c, e1, e2, s, persp, id);
           e1 = e1 + s.n1x;
           e2 = e2 + s.n2x;
         }
-      }
-      __syncthreads();
-      // Edge hand-over: each listed item walks its row again and stores its
-      // edge values at the pixels it currently wins.  A pixel has one winner,
-      // so every pixel has at most one writer; a later round that takes the
-      // pixel over unsets the values in its raster pass first.
-      const int nh = HO ? min(hcount, QCAP) : 0;
-      for (int h = tid; h < nh; h += NT) {
-        const Handoff hd = hlist[h];
-        const TriSetup& s = setups[hd.slot];
-        const int cx0 = max(s.bx0, x0), cx1 = min(s.bx1, x0 + TW - 1);
-        const int prow = (hd.row - y0) * KS - x0;
-        const uint32_t id = setup_id[hd.slot];
-        const uint32_t* key_lo = reinterpret_cast<const uint32_t*>(keys);
-        float e1 = hd.e1, e2 = hd.e2;
-        for (int c = cx0; c <= cx1; c++) {
-          if (key_lo[2 * (prow + c)] == id) ebuf[prow + c] = make_float2(e1, e2);
-          e1 = e1 + s.n1x;
-          e2 = e2 + s.n2x;
-        }
-      }
-      if (HO) {
-        __syncthreads();
-        if (tid == 0) hcount = 0;
       }
       __syncthreads();
     }
@@ -810,7 +754,6 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
         if (c < J.W && row < J.H) {
           const size_t pix = (size_t)row * J.W + c;
           J.win[pix] = keys[lp] == ~0ull ? kNoWinner : (uint32_t)keys[lp];
-          if (HO) reinterpret_cast<float2*>(J.edge)[pix] = ebuf[lp];
         }
       }
     } else {
@@ -841,10 +784,8 @@ __global__ __launch_bounds__(NT) void k_tile(BatchArgs a) {
             const int c = x0 + (p % TW), row = y0 + (p / TW);
             const u64 k = __hip_atomic_load(&gk[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (c < J.W && row < J.H) {
-              // the parts' edge values are not merged: shading walks again
               const size_t pix = (size_t)row * J.W + c;
               J.win[pix] = k == ~0ull ? kNoWinner : (uint32_t)k;
-              if (HO) reinterpret_cast<uint2*>(J.edge)[pix] = make_uint2(kEdgeUnset, kEdgeUnset);
             }
           }
         }
@@ -868,7 +809,7 @@ __device__ inline f3 fast_norm3(f3 a) { return scale3(fast_rlen3(a), a); }
 
 // Shading of a covered pixel whose winner `id` belongs to draw D.
 __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, const DrawDesc& D,
-                                          uint32_t id, float2 ev, int c, int row, size_t pix) {
+                                          uint32_t id, int c, int row, size_t pix) {
   const int64_t g = id >> 1;
   const int sub = (int)(id & 1u);
   const int tri = (int)(g - D.tri_begin);
@@ -905,24 +846,22 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   v2.p = ec[2];
   TriSetup s;
   tri_setup_corners(J, ec, vc, sub, s);
-  // Edge values at the pixel, all from the same serial walk along the row:
-  // handed over by the raster, or walked from the raster's checkpoint where
-  // the row enters the pixel's tile column, or from the bbox's left column.
-  float e1 = ev.x, e2 = ev.y;
-  if (!(e1 == e1)) {
-    const int sw = __ffs(a.tile_w) - 1;
-    const int btx = s.bx0 >> sw, ctx = c >> sw;
-    if (cb != kNoCkpt && ctx > btx) {
-      const float2 st =
-          a.ckpt[cb + (uint32_t)((row - s.by0) * ((s.bx1 >> sw) - btx) + (ctx - btx - 1))];
-      e1 = st.x;
-      e2 = st.y;
-      advance2(e1, s.n1x, e2, s.n2x, c - (ctx << sw));
-    } else {
-      edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
-                 (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
-      advance2(e1, s.n1x, e2, s.n2x, c - s.bx0);
-    }
+  // Edge values at the pixel, from the same serial walk along the row as the
+  // raster's: from the raster's checkpoint where the row enters the pixel's
+  // tile column, or from the bbox's left column.
+  float e1, e2;
+  const int sw = __ffs(a.tile_w) - 1;
+  const int btx = s.bx0 >> sw, ctx = c >> sw;
+  if (cb != kNoCkpt && ctx > btx) {
+    const float2 st =
+        a.ckpt[cb + (uint32_t)((row - s.by0) * ((s.bx1 >> sw) - btx) + (ctx - btx - 1))];
+    e1 = st.x;
+    e2 = st.y;
+    advance2(e1, s.n1x, e2, s.n2x, c - (ctx << sw));
+  } else {
+    edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
+               (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
+    advance2(e1, s.n1x, e2, s.n2x, c - s.bx0);
   }
   float k[3];
   const float z = depth_of(e1, e2, s.z0, s.z1, s.z2, J.persp != 0, k);
@@ -1003,7 +942,7 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
 
 // Deferred shading of one pixel from its winner (Renderer.cpp:271-305).
 __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
-                            const int64_t* dcache, int ncached, uint32_t id, float2 ev,
+                            const int64_t* dcache, int ncached, uint32_t id,
                             int c, int row) {
   const size_t pix = (size_t)row * (size_t)J.W + (size_t)c;
   if (id == kNoWinner) {  // clear(), Renderer.cpp:113-119
@@ -1032,9 +971,9 @@ __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
   if (__ballot(d == d0) == __ballot(1)) {
     // One draw for the whole wavefront: its descriptor through scalar loads.
     const DrawDesc Du = uniform_draw(a, d0);
-    shade_lit(a, J, Du, id, ev, c, row, pix);
+    shade_lit(a, J, Du, id, c, row, pix);
   } else {
-    shade_lit(a, J, a.draws[d], id, ev, c, row, pix);
+    shade_lit(a, J, a.draws[d], id, c, row, pix);
   }
 }
 
@@ -1082,13 +1021,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
   const int r = tid / kShadeW;
   const int c = (local - byi * bxn) * kShadeW + (tid - r * kShadeW), row = byi * kShadeH + r;
   const bool inside = c < J.W && row < J.H;
-  // The winner and edge loads are issued first, so that their latency
-  // overlaps the draw-table fill below.
+  // The winner load is issued first, so that its latency overlaps the
+  // draw-table fill below.
   const size_t pix = (size_t)row * J.W + c;
   const uint32_t id = inside ? J.win[pix] : kNoWinner;
-  const float2 ev = inside && a.edge_handoff
-                        ? reinterpret_cast<const float2*>(J.edge)[pix]
-                        : make_float2(__uint_as_float(kEdgeUnset), 0.0f);
   // A job of one draw needs no draw table (ncached = -1).
   const bool one_draw = J.draw_end - J.draw_begin == 1;
   const int ncached = one_draw ? -1 : fill_draw_cache(dcache, a, J, tid, kShadeBlock);
@@ -1104,7 +1040,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
   }
   if (!one_draw) __syncthreads();
   if (blockIdx.x == 0) reduce_stats(a, tid);
-  if (inside) shade_pixel(a, J, dcache, ncached, id, ev, c, row);
+  if (inside) shade_pixel(a, J, dcache, ncached, id, c, row);
 }
 
 // ---- launchers -------------------------------------------------------------
@@ -1157,10 +1093,7 @@ bool raster_threads_supported(int tw, int nt) {
 
 #define MR_TILE_CASE(TW, NT)                                                         \
   if (a.tile_w == TW && a.raster_threads == NT) {                                     \
-    if (a.edge_handoff)                                                               \
-      hipLaunchKernelGGL((k_tile<TW, TW, NT, true>), dim3(grid), dim3(NT), 0, s, a);  \
-    else                                                                              \
-      hipLaunchKernelGGL((k_tile<TW, TW, NT, false>), dim3(grid), dim3(NT), 0, s, a); \
+    hipLaunchKernelGGL((k_tile<TW, TW, NT>), dim3(grid), dim3(NT), 0, s, a);          \
     return hipGetLastError();                                                         \
   }
 

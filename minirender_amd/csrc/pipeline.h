@@ -52,7 +52,6 @@ struct BatchArgs {
   int32_t ntiles;
   int32_t tile_w, tile_h;    // each 8, 16 or 32
   int32_t raster_threads;    // workgroup size of the tile raster
-  int32_t edge_handoff;      // raster hands winners' edge values to shading
   int32_t scan_blocks;
   int32_t shade_blocks;
   int32_t num_cus;

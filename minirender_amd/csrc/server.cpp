@@ -56,12 +56,12 @@ struct Flight {
   int64_t batch_id = -1;
 };
 
-// Device memory a request holds while in flight: outputs, winner ids and
-// edge values, per-triangle and per-tile scratch (items 32 B, bins ~12 B,
+// Device memory a request holds while in flight: outputs, winner ids, edge
+// checkpoints, per-triangle and per-tile scratch (items 32 B, bins ~12 B,
 // regions; tile counters 2 x 64 B + 16 B), a conservative estimate.
 uint64_t request_bytes(mr_ctx* c, const mr_frame& f, const mr_outputs& o) {
   const uint64_t npix = (uint64_t)f.width * f.height;
-  uint64_t b = npix * 28;  // winner ids, edge values, edge checkpoints
+  uint64_t b = npix * 20;  // winner ids, edge checkpoints
   if (o.rgb8) b += npix * 3;
   if (o.depth) b += npix * 4;
   if (o.winner) b += npix * 4;

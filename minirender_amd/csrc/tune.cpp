@@ -1,8 +1,7 @@
 // tune.cpp -- autotuner of the launch configuration (C-ABI mr_ctx_autotune).
 //
 // For a representative batch of frames it times every built combination of
-// tile size, raster workgroup size, split size of crowded tiles and edge
-// hand-over on or off,
+// tile size, raster workgroup size and split size of crowded tiles,
 // with HIP events on the context's stream, keeps the fastest, and sizes the
 // bin list from the measured (tile, triangle) pairs per triangle.  Results
 // can be kept in a small text cache keyed by the GPU and the batch's shape
@@ -60,8 +59,8 @@ bool cache_lookup(const char* path, const std::string& key, mr_tuning& out) {
   while (fgets(line, sizeof(line), f)) {
     char k[256];
     mr_tuning t{};
-    if (sscanf(line, "%255s %d %d %f %d %d %lf", k, &t.tile, &t.raster_threads, &t.bin_factor,
-               &t.frag_size, &t.edge_handoff, &t.ms) == 7 &&
+    if (sscanf(line, "%255s %d %d %f %d %lf", k, &t.tile, &t.raster_threads, &t.bin_factor,
+               &t.frag_size, &t.ms) == 6 &&
         key == k) {
       out = t;  // the last entry for a key wins
       found = true;
@@ -75,8 +74,8 @@ void cache_store(const char* path, const std::string& key, const mr_tuning& t) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %.3f %d %d %.6f\n", key.c_str(), t.tile, t.raster_threads, t.bin_factor,
-          t.frag_size, t.edge_handoff, t.ms);
+  fprintf(f, "%s %d %d %.3f %d %.6f\n", key.c_str(), t.tile, t.raster_threads, t.bin_factor,
+          t.frag_size, t.ms);
   fclose(f);
 }
 
@@ -95,7 +94,6 @@ int mr_ctx_autotune(mr_ctx* c, const mr_frame* frames, int32_t n, const mr_outpu
       mr_ctx_set_raster_threads(c, pick.raster_threads) == MR_OK &&
       mr_ctx_set_bin_factor(c, pick.bin_factor) == MR_OK && pick.frag_size >= 64) {
     c->frag_size = (uint32_t)pick.frag_size;
-    c->edge_handoff = pick.edge_handoff != 0;
     pick.from_cache = 1;
     if (best) *best = pick;
     return MR_OK;
@@ -112,25 +110,22 @@ int mr_ctx_autotune(mr_ctx* c, const mr_frame* frames, int32_t n, const mr_outpu
   const float bf_saved = c->bin_factor;
   c->bin_factor = std::max(bf_saved, 4.0f);
   const uint32_t frag_saved = c->frag_size;
-  const bool ho_saved = c->edge_handoff;
-  struct Cand { int tile, nt, frag, ho; };
+  struct Cand { int tile, nt, frag; };
   std::vector<Cand> cands;
   const int tiles[3] = {8, 16, 32};
   const int nts[4] = {64, 128, 256, 512};
   for (int t : tiles)
     for (int nt : nts)
       if (raster_threads_supported(t, nt))
-        for (int frag : {256, 1024, 4096})
-          for (int ho : {0, 1}) cands.push_back({t, nt, frag, ho});
+        for (int frag : {256, 1024, 4096}) cands.push_back({t, nt, frag});
   int rc = MR_OK;
   double best_ms = 1e300;
-  Cand best_c = {16, 256, 1024, 0};
+  Cand best_c = {16, 256, 1024};
   int64_t pairs = 0, tris = 0;
   for (const Cand& cd : cands) {
     if ((rc = mr_ctx_set_tile(c, cd.tile, cd.tile))) break;
     if ((rc = mr_ctx_set_raster_threads(c, cd.nt))) break;
     c->frag_size = (uint32_t)cd.frag;
-    c->edge_handoff = cd.ho != 0;
     if ((rc = render_frames(c, frames, n, outs))) break;  // warm: scratch sized
     std::vector<double> ts;
     for (int r = 0; r < reps && rc == MR_OK; r++) {
@@ -158,13 +153,11 @@ int mr_ctx_autotune(mr_ctx* c, const mr_frame* frames, int32_t n, const mr_outpu
   if (rc) {
     c->bin_factor = bf_saved;
     c->frag_size = frag_saved;
-    c->edge_handoff = ho_saved;
     return rc;
   }
   mr_ctx_set_tile(c, best_c.tile, best_c.tile);
   mr_ctx_set_raster_threads(c, best_c.nt);
   c->frag_size = (uint32_t)best_c.frag;
-  c->edge_handoff = best_c.ho != 0;
   // Bin capacity: the measured pairs per triangle with 25% headroom.
   float bf = tris > 0 ? (float)((double)pairs / (double)tris * 1.25) : bf_saved;
   bf = std::min(std::max(bf, 1.0f), 1024.0f);
@@ -172,7 +165,6 @@ int mr_ctx_autotune(mr_ctx* c, const mr_frame* frames, int32_t n, const mr_outpu
   pick.tile = best_c.tile;
   pick.raster_threads = best_c.nt;
   pick.frag_size = best_c.frag;
-  pick.edge_handoff = best_c.ho;
   pick.bin_factor = bf;
   pick.ms = best_ms;
   pick.from_cache = 0;

@@ -52,51 +52,49 @@ def test_fuzz_scenes(gpu_ctx, seed):
     check(gpu_ctx, fuzz_frame(seed), f"fuzz seed {seed}")
 
 
-@pytest.mark.parametrize("handoff", [False, True])
 @pytest.mark.parametrize("tile,threads", [(8, 64), (8, 128), (16, 64), (16, 128), (16, 256),
                                           (16, 512), (32, 64), (32, 128), (32, 256), (32, 512)])
-def test_tile_sizes(gpu_ctx, tile, threads, handoff):
+def test_tile_sizes(gpu_ctx, tile, threads):
     gpu_ctx.set_tile(tile, tile)
     gpu_ctx.set_raster_threads(threads)
-    gpu_ctx.set_edge_handoff(handoff)
     try:
-        what = f"tile {tile}/{threads} handoff {handoff}"
+        what = f"tile {tile}/{threads}"
         for seed in (100, 101, 102):
             check(gpu_ctx, fuzz_frame(seed, 133, 71), f"{what} seed {seed}")
         # split tiles (more pairs than one raster work item) with this shape
         check(gpu_ctx, stack_frame(1500, 80, 56, seed=tile + threads), f"{what} split")
     finally:
         gpu_ctx.set_tile(16, 16)
-        gpu_ctx.set_edge_handoff(False)
 
 
-@pytest.mark.parametrize("tile", [16, 32])
-def test_edge_handoff_large_triangles(gpu_ctx, tile):
-    """Shading from the raster's handed-over edge values: large and clipped
-    triangles (long rows, many tiles per triangle, split tiles), textures."""
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_edge_checkpoints_large_triangles(gpu_ctx, tile):
+    """Shading walks from the raster's edge checkpoints: large and clipped
+    triangles (long rows, many tile columns per triangle), textures, tiles of
+    several raster rounds, split tiles, and a frame whose checkpoints do not
+    all fit (those rows are walked from the bbox)."""
     gpu_ctx.set_tile(tile, tile)
-    gpu_ctx.set_edge_handoff(True)
     try:
         rng_tris = scenes.random_triangles(40, 21, lo=0.2, hi=1.5, center_lo=(-1, -1, -3),
                                            center_hi=(1, 1, -2))
         fr = FrameSpec(640, 360, scenes.projection_frustum(math.radians(60), 16 / 9, 0.1, 50.0),
                        draws=[Draw(rng_tris, Material(shininess=40.0))], light=(0.3, 0.5, 1.0))
-        check(gpu_ctx, fr, f"handoff large triangles t{tile}")
+        check(gpu_ctx, fr, f"checkpoints large triangles t{tile}")
         check(gpu_ctx, scenes.config_textured(n=80, width=320, height=180, ntris=300, tex_size=256),
-              f"handoff textured t{tile}")
+              f"checkpoints textured t{tile}")
         check(gpu_ctx, scenes.config_terrain(n=96, width=320, height=180, shapes=True),
-              f"handoff terrain t{tile}")
+              f"checkpoints terrain t{tile}")
         # tiles of several rounds in one workgroup: later rounds take pixels
         # over from earlier ones
         gpu_ctx.set_raster_threads(64)
-        check(gpu_ctx, stack_frame(300, 64, 48, seed=4), f"handoff rounds t{tile}")
-        # a crowded frame whose tiles are split into parts (no hand-over there)
+        check(gpu_ctx, stack_frame(300, 64, 48, seed=4), f"checkpoints rounds t{tile}")
+        # crowded tiles split into parts; at 800 screen-covering layers the
+        # checkpoint buffer (2 per pixel + 64 Ki) overflows
         gpu_ctx.set_frag_size(64)
-        check(gpu_ctx, stack_frame(800, 64, 48, seed=5), f"handoff split t{tile}")
+        check(gpu_ctx, stack_frame(800, 64, 48, seed=5), f"checkpoints split t{tile}")
     finally:
         gpu_ctx.set_frag_size(1024)
         gpu_ctx.set_tile(16, 16)
-        gpu_ctx.set_edge_handoff(False)
 
 
 def test_terrain_clipped(gpu_ctx):
@@ -217,7 +215,7 @@ def test_autotune_and_cache(tmp_path):
             check(ctx, fr, f"autotuned frame {i}")
         t2 = ctx.autotune(frames, reps=1, cache_path=cache)
         assert t2["from_cache"]
-        for k in ("tile", "raster_threads", "frag_size", "edge_handoff"):
+        for k in ("tile", "raster_threads", "frag_size"):
             assert t2[k] == t1[k]
     finally:
         ctx.close()
