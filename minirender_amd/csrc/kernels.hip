@@ -3,31 +3,35 @@
 //
 //   k_clear   look-back words, tile counts and merge counters, when the
 //             scratch is new (afterwards the shading kernel leaves them zero)
-//   k_geom    (a)+(b) vertex transform, near clip, projection, setup and
-//             cull of every input triangle (Renderer.cpp:340-380, 163-236).
-//             Each wavefront compacts its survivors into its own region of
-//             the item stream (ballot + popcount prefix, no global atomic)
-//             and counts its (tile, triangle) pairs with one atomic per
-//             distinct tile
+//   k_vertex  (a) eye-space transform and projection once per vertex of each
+//             (mesh, modelview) group of a job (Renderer.cpp:340-349, 191-203)
+//   k_geom    (a)+(b) index gather, near clip, setup and cull of every input
+//             triangle (Renderer.cpp:351-380, 163-236).  Each wavefront
+//             compacts its survivors into its own region of the item stream
+//             (ballot + popcount prefix, no global atomic) and counts its
+//             (tile, triangle) pairs with one atomic per distinct tile
 //   k_scan    (c) exclusive scan of the tile counts (decoupled look-back);
 //             tiles with more pairs than one work item are split into parts
-//   k_bin     (c) scatter of triangle ids into the tiles' bin lists: per
-//             wavefront step, one atomic instruction for all distinct tiles
+//   k_bin     (c) scatter of (triangle id, draw) into the tiles' bin lists:
+//             per wavefront step, one atomic instruction for all distinct
+//             tiles; takes the edge checkpoints of triangles spanning
+//             several tile columns
 //   k_tile    (d) per-tile raster with the tile's 64-bit
 //             (depth key, draw-order id) buffer in LDS and ds_min_u64; parts
 //             of a split tile merge through global 64-bit atomicMin.  Writes
-//             each pixel's winning id
+//             each pixel's winning id and the rows' edge checkpoints
 //   k_shade   (e) deferred Blinn-Phong + texture, one lane per pixel in image
-//             order, writing RGB8 / depth / winner / float colour / normals
+//             order, writing RGB8 / depth / winner / float colour / normals /
+//             range image
 //
 // A triangle's id is 2 * (flat triangle index in the batch) + clip sub-index,
 // which increases with draw order inside a job.  The minimum of
 // (depth key, id) is therefore the first-drawn triangle of minimum depth --
 // the winner of the reference's sequential strict `z < depth`
-// (Renderer.cpp:267).  Setup is recomputed from the resident mesh wherever
-// it is needed (binning, raster, shading) instead of being stored: the same
-// fp32 operations on the same inputs give the same bits, and ALU is cheaper
-// than HBM traffic.
+// (Renderer.cpp:267).  Triangle setup is recomputed from the projected
+// vertices wherever it is needed (geometry, raster, shading) instead of being
+// stored: the same fp32 operations on the same inputs give the same bits, and
+// ALU is cheaper than writing and re-reading a record per triangle.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
