@@ -1,8 +1,7 @@
 // io.h -- files of the minirender API (reference: include/minirender/io.h:
 // 10-22): PPM (P6) save with the reference's RGB8 quantization and load with
-// texel = byte / 255; STL (ASCII / binary) and OBJ (+ MTL) loaders, STL and
-// XYZ point writers.  X3D scenes are not read (loadMesh returns an empty
-// node for them).
+// texel = byte / 255; STL (ASCII / binary), OBJ (+ MTL) and X3D loaders,
+// STL and XYZ point writers.
 #ifndef MINIRENDER_IO_H
 #define MINIRENDER_IO_H
 
@@ -17,7 +16,8 @@ namespace minirender {
 bool savePPM(const Array2<Vec3>& image, const std::string& filename);
 Array2<Vec3> loadPPM(const std::string& filename);
 
-// io.cpp:114-133: .stl -> a node holding the mesh, .obj -> loadOBJ.
+// io.cpp:114-133: .stl -> a node holding the mesh, .obj -> loadOBJ,
+// .x3d -> loadX3D, anything else -> an empty node.
 Shared<SceneNode> loadMesh(const std::string& filename);
 // io.cpp:135-152 (binary when the size is 84 + 50 * facets).
 Shared<TriMesh> loadSTL(const std::string& filename);
@@ -28,6 +28,9 @@ Shared<SceneNode> loadOBJ(const std::string& filename);
 // io.cpp:417-431: points with z < -1e-5, transformed by m, "%f %f %f" lines.
 bool saveXYZ(const Array2<Vec3>& points, const std::string& filename,
              const Matrix4& m = Matrix4::identity());
+// x3d.cpp:35-196: Transform/Group/Shape/Inline scene of an X3D file (null if
+// it is not an X3D document with a Scene).
+Shared<SceneNode> loadX3D(const std::string& filename);
 // x3d.cpp:17-33: fan triangulation of -1-terminated polygons.
 std::vector<int> triangulateIndices(const std::vector<int>& indices);
 

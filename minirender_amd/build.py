@@ -29,7 +29,7 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
 ]
 SOURCES = ["kernels.hip", "context.cpp", "projection.cpp", "cpp_api.cpp", "io.cpp",
-           "server.cpp", "tune.cpp"]
+           "x3d.cpp", "server.cpp", "tune.cpp"]
 
 
 def _compile(src: Path) -> Path:

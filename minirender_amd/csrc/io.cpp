@@ -297,7 +297,8 @@ Shared<SceneNode> loadMesh(const std::string& filename) {
     return node;
   }
   if (has_extension(filename, "obj")) return loadOBJ(filename);
-  return std::make_shared<SceneNode>();  // X3D and other formats: not read by this library
+  if (has_extension(filename, "x3d")) return loadX3D(filename);
+  return std::make_shared<SceneNode>();
 }
 
 bool saveXYZ(const Array2<Vec3>& points, const std::string& filename, const Matrix4& m) {

@@ -105,6 +105,67 @@ static int loader_checks() {
     Shared<SceneNode> node = loadMesh("/tmp/mr_api_check_a.stl");
     CHECK(node && node->children.size() == 1, "loadMesh stl");
   }
+  // X3D: a Transform holding a Shape (IndexedFaceSet: a quad and a
+  // triangle, commas, no normals or texcoords, DEF appearance), a Group
+  // with a Shape that USEs the appearance (IndexedTriangleSet with normals
+  // and texcoords), and an Inline scene; comments, prolog and DOCTYPE.
+  {
+    FILE* in = std::fopen("/tmp/mr_api_check_inl.x3d", "w");
+    std::fprintf(in,
+                 "<X3D><Scene><Shape><IndexedTriangleSet index='0 1 2'>"
+                 "<Coordinate point='0 0 0 1 0 0 0 1 0'/></IndexedTriangleSet></Shape>"
+                 "</Scene></X3D>\n");
+    std::fclose(in);
+    FILE* x = std::fopen("/tmp/mr_api_check.x3d", "w");
+    std::fprintf(x,
+                 "<?xml version=\"1.0\" encoding=\"UTF-8\"?>\n"
+                 "<!DOCTYPE X3D PUBLIC \"ISO//Web3D//DTD X3D 3.3//EN\" \"x3d-3.3.dtd\">\n"
+                 "<X3D profile='Interchange'><head><meta name='t' content='a &amp; b'/></head>\n"
+                 "<Scene><!-- a comment <Shape/> -->\n"
+                 "<Transform translation='1 2 3' rotation='0 0 2 1.5707963' scale='2 2 2'>\n"
+                 " <Shape><Appearance DEF='look'><Material diffuseColor='0.1 0.2 0.3' "
+                 "shininess='0.25'/></Appearance>\n"
+                 "  <IndexedFaceSet coordIndex='0 1 2 3 -1, 0 2 4 -1'>"
+                 "<Coordinate point='0 0 0, 1 0 0, 1 1 0, 0 1 0, 2 2 2'/></IndexedFaceSet>\n"
+                 " </Shape>\n"
+                 "</Transform>\n"
+                 "<Group><Shape><Appearance USE='look'/>\n"
+                 "  <IndexedTriangleSet index='0 1 2'><Coordinate point='0 0 0 1 0 0 0 1 0'/>"
+                 "<Normal vector='0 0 1 0 0 1 0 0 1'/>"
+                 "<TextureCoordinate point='0 0 1 0 0.25 0.75'/></IndexedTriangleSet>\n"
+                 "</Shape></Group>\n"
+                 "<Inline url='\"mr_api_check_inl.x3d\"'/>\n"
+                 "</Scene></X3D>\n");
+    std::fclose(x);
+    Shared<SceneNode> root = loadMesh("/tmp/mr_api_check.x3d");
+    CHECK(root && root->children.size() == 3, "X3D root children");
+    SceneNode* tr = root->children[0].get();
+    CHECK(tr && tr->children.size() == 1, "X3D transform children");
+    // translate(1,2,3) * rotate(z, pi/2) * scale(2): (1,0,0) -> (1, 4, 3)
+    const Vec3 p = tr->transform * Vec3(1, 0, 0);
+    CHECK(std::fabs(p.x - 1) < 1e-5f && std::fabs(p.y - 4) < 1e-5f && std::fabs(p.z - 3) < 1e-5f,
+          "X3D transform %f %f %f", p.x, p.y, p.z);
+    auto* m0 = dynamic_cast<TriMesh*>(tr->children[0].get());
+    CHECK(m0 && m0->indices == std::vector<int>({0, 1, 2, 0, 2, 3, 0, 2, 4}), "X3D faces");
+    CHECK(m0->normalsI == std::vector<int>({0, 0, 0, 1, 1, 1, 2, 2, 2}) &&
+              m0->normals.size() == 3 && m0->normals[0].z == 1.0f,
+          "X3D face normals");
+    CHECK(m0->texcoords.size() == 1 && m0->texcoordsI == std::vector<int>(9, 0),
+          "X3D default texcoords");
+    CHECK(m0->material && m0->material->diffuse.y == 0.2f && m0->material->shininess == 2.0f &&
+              m0->material->specular.x == 0.4f,
+          "X3D material");
+    auto* m1 = dynamic_cast<TriMesh*>(root->children[1]->children[0].get());
+    CHECK(m1 && m1->material && m1->material->diffuse.z == 0.3f, "X3D USE appearance");
+    CHECK(m1->indices == std::vector<int>({0, 1, 2}) && m1->normalsI == m1->indices &&
+              m1->normals.size() == 3 && m1->texcoords.size() == 3 &&
+              m1->texcoords[2].y == 0.25f,
+          "X3D triangle set");
+    SceneNode* inl = root->children[2].get();
+    CHECK(inl && inl->children.size() == 1 && dynamic_cast<TriMesh*>(inl->children[0].get()),
+          "X3D inline");
+    CHECK(!loadX3D("/tmp/mr_api_check_a.stl"), "X3D rejects a non-X3D file");
+  }
   return 0;
 }
 
