@@ -349,7 +349,6 @@ def serving_section(args, torch, mra):
     log(f"serving: pool of {len(pool)} meshes, {len(reqs)} requests, "
         f"generated in {time.perf_counter() - t_gen:.1f} s")
     ctx = mra.Context(0)
-    ctx.set_tile(16, 16)
     dev = torch.device("cuda:0")
     # Framebuffers, one set per resolution shared by all requests of that
     # resolution (the service completes them in order): device-resident
@@ -381,7 +380,23 @@ def serving_section(args, torch, mra):
         a = np.cumsum(unit_gaps[:count]) / rate
         return a - a[0]
 
-    def trace(max_batch, slots, arr, count, where):
+    # Launch configuration of each mode, tuned on its own shape: the plain
+    # service on one request of median size, the batched one on a batch of
+    # requests (a service tunes once per GPU and workload class).
+    k = min(n, args.serve_batch)
+    med = sorted(range(k), key=lambda i: reqs[i].ntri)[k // 2]
+    tunes = {"plain": ctx.autotune([reqs[med]], outs=[o_dev[med]], reps=5),
+             "batched": ctx.autotune(reqs[:k], outs=o_dev[:k], reps=2)}
+    log(f"serving: {tunes}")
+    modes = {"plain": (1, 1), "batched": (args.serve_batch, args.serve_slots)}
+
+    def trace(mode, arr, count, where):
+        cfg = tunes[mode]
+        ctx.set_tile(cfg["tile"], cfg["tile"])
+        ctx.set_raster_threads(cfg["raster_threads"])
+        ctx.set_frag_size(cfg["frag_size"])
+        ctx.set_bin_factor(cfg["bin_factor"])
+        max_batch, slots = modes[mode]
         with mra.Server(ctx, max_batch=max_batch, slots=slots,
                         hbm_budget=32 * (1 << 30)) as srv:
             lat, mk = srv.run_trace(farr, list(arr[:count]), oarr[where])
@@ -390,33 +405,32 @@ def serving_section(args, torch, mra):
                 "frames_per_s": round(count / mk, 1), "requests": count,
                 "offered_rate": round(float(count / max(arr[count - 1], 1e-9)), 1)}
 
-    modes = {"plain": (1, 1), "batched": (args.serve_batch, args.serve_slots)}
-    trace(args.serve_batch, args.serve_slots, np.zeros(n), min(n, 1024), "device")  # warm
+    trace("batched", np.zeros(n), min(n, 1024), "device")  # warm
     out = {"rate": args.serve_rate, "pool_meshes": len(pool), "hbm_budget_gb": 32,
-           "p99_budget_ms": args.p99_budget,
+           "p99_budget_ms": args.p99_budget, "autotune": tunes,
            "modes": {k: {"max_batch": v[0], "slots": v[1]} for k, v in modes.items()}}
     for where in ("device", "host"):
         res = {}
         count = n if where == "device" else min(n, 10000)
-        for name, (mb, sl) in modes.items():
-            res[name] = trace(mb, sl, arrivals(args.serve_rate, count), count, where)
+        for name in modes:
+            res[name] = trace(name, arrivals(args.serve_rate, count), count, where)
             log(f"serving {where} {name}: {res[name]}")
         out[where] = res
     # Highest offered rate each mode sustains with p99 within the budget
     # (device framebuffers): doubling from 500 req/s, then three bisections.
-    def sustains(mb, sl, rate):
+    def sustains(mode, rate):
         count = int(min(n, max(2000, rate * 1.5)))
-        r = trace(mb, sl, arrivals(rate, count), count, "device")
+        r = trace(mode, arrivals(rate, count), count, "device")
         r["rate"] = round(rate, 1)
         ok = r["p99_ms"] <= args.p99_budget and r["frames_per_s"] >= 0.9 * rate
         log(f"sweep {r}: {'ok' if ok else 'over'}")
         return ok, r
 
     cap = {}
-    for name, (mb, sl) in modes.items():
+    for name in modes:
         best, rate, fail = None, 500.0, None
         while rate <= 512000:
-            ok, r = sustains(mb, sl, rate)
+            ok, r = sustains(name, rate)
             if not ok:
                 fail = rate
                 break
@@ -425,7 +439,7 @@ def serving_section(args, torch, mra):
             lo, hi = best["rate"], fail
             for _ in range(3):
                 mid = (lo + hi) / 2
-                ok, r = sustains(mb, sl, mid)
+                ok, r = sustains(name, mid)
                 if ok:
                     best, lo = r, mid
                 else:
