@@ -823,9 +823,20 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   const int* in = D.nidx + 3 * tri;
   const uint32_t cb = a.ckpt_base[id];
   const int i0 = ip[0], i1 = ip[1], i2 = ip[2];
-  const int n0 = in[0], n1 = in[1], n2 = in[2];
+  // Normal and texcoord indices that share the position index buffer are
+  // not loaded again.
+  int n0 = i0, n1 = i1, n2 = i2;
+  if (D.nidx != D.idx) {
+    n0 = in[0];
+    n1 = in[1];
+    n2 = in[2];
+  }
   int t0 = 0, t1 = 0, t2 = 0;
-  if (D.tidx) {
+  if (D.tidx == D.idx) {
+    t0 = i0;
+    t1 = i1;
+    t2 = i2;
+  } else if (D.tidx) {
     const int* it = D.tidx + 3 * tri;
     t0 = it[0];
     t1 = it[1];

@@ -118,6 +118,29 @@ def test_fullscreen_triangles(gpu_ctx):
     check(gpu_ctx, fr, "fullscreen triangles", ("rgb8", "depth", "winner"))
 
 
+def test_separate_index_arrays(gpu_ctx):
+    """Normal and texcoord index arrays of their own (face normals, permuted
+    texcoords), as OBJ files give them: shading reads them instead of the
+    position indices."""
+    from minirender_amd.scenes import Mesh
+    rng = np.random.default_rng(17)
+    sph = scenes.create_sphere(1.0, 12, 24)
+    pos, idx = sph.positions, sph.indices
+    a, b, c = pos[idx[:, 0]], pos[idx[:, 1]], pos[idx[:, 2]]
+    fn = np.cross(b - a, c - a).astype(np.float32)
+    fn /= np.maximum(np.linalg.norm(fn, axis=1, keepdims=True), 1e-12).astype(np.float32)
+    nidx = np.repeat(np.arange(len(idx), dtype=np.int32)[:, None], 3, axis=1)
+    uvs = rng.uniform(0.0, 1.0, (97, 2)).astype(np.float32)
+    uidx = rng.integers(0, len(uvs), idx.shape).astype(np.int32)
+    mesh = Mesh(pos, fn, idx, normal_indices=nidx, uvs=uvs, uv_indices=uidx)
+    mat = Material(shininess=20.0, texture=scenes.checker_texture(64, 3, 8))
+    fr = FrameSpec(200, 150, scenes.projection_frustum(math.radians(45), 4 / 3, 0.1, 50.0),
+                   view=scenes.translate(0.0, 0.0, -3.0), draws=[Draw(mesh, mat)],
+                   light=(1.0, 2.0, 3.0), light_is_point=True)
+    g, o = check(gpu_ctx, fr, "separate index arrays")
+    assert (g["winner"] != 0xFFFFFFFF).sum() > 5000
+
+
 def test_bin_overflow_retry(gpu_ctx):
     fr = fuzz_frame(7, 640, 480)
     gpu_ctx.set_bin_factor(1.0)
