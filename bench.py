@@ -185,9 +185,10 @@ def plain_section(args, torch, mra, frame, wl_name):
         tune = ctx.autotune([frame], outs=[outs], reps=3, cache_path=args.tune_cache)
     tile = tune["tile"]
     log(f"plain: {tune}")
-    # Timed steps: the steady state as a service runs it (graph replay of an
-    # unchanged frame's launches), no per-stage events.
-    ctx.set_graphs(True)
+    # Timed steps: synchronous mr_render calls with eager launches (on this
+    # GPU and ROCm, replaying the frame as a captured HIP graph costs more
+    # host time per frame than the seven launches it replaces), no per-stage
+    # events.
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -492,7 +493,7 @@ def main():
         "config": {"workload": wl, "triangles": frame.ntri, "width": frame.width,
                    "height": frame.height, "tile": plain["tile"],
                    "autotune": plain["autotune"],
-                   "path": "plain mr_render, steady state replayed as a HIP graph",
+                   "path": "plain synchronous mr_render, eager launches",
                    "outputs": "device (RGB8 + fp32 depth)"},
         "roofline": plain["roofline"],
         "kernels": plain["kernels"],
