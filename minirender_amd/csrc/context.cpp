@@ -338,6 +338,8 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.scan_blocks = scan_blocks_for((int32_t)p.ntiles);
   a.shade_blocks = (int32_t)p.shade_blocks;
   a.num_cus = c->num_cus;
+  a.geom_blocks = std::min(std::max(geom_blocks_per_cu(), 1) * std::max(c->num_cus, 1),
+                           kStatSlots);
   return a;
 }
 

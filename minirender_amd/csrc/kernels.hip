@@ -1067,6 +1067,18 @@ hipError_t launch_clear(const BatchArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+int geom_blocks_per_cu() {
+  static const int n = [] {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_geom, 256, 0) != hipSuccess || b < 1) {
+      (void)hipGetLastError();
+      b = 8;
+    }
+    return b;
+  }();
+  return n;
+}
+
 hipError_t launch_vertex(const BatchArgs& a, hipStream_t s) {
   if (a.total_verts == 0) return hipSuccess;
   const int64_t blocks = (a.total_verts + 255) / 256;

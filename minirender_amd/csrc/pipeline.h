@@ -55,6 +55,7 @@ struct BatchArgs {
   int32_t scan_blocks;
   int32_t shade_blocks;
   int32_t num_cus;
+  int32_t geom_blocks;       // resident geometry workgroups (<= kStatSlots)
 };
 
 constexpr int kStatSlots = 8192;  // >= the geometry and bin grids
@@ -64,10 +65,19 @@ inline int32_t scan_blocks_for(int32_t tiles) {
 }
 
 // Grid sizes of the grid-stride kernels (also used to sum their statistics).
+// Workgroups of the geometry kernel, each taking one contiguous range of
+// triangles: as many as the GPU holds at once (geom_blocks, from the
+// kernel's occupancy), so none waits for a second round; a batch of more than
+// 4096 triangles per such workgroup gets ~1024-triangle ranges instead, whose
+// rounds even out the uneven draws of many jobs.
 __host__ __device__ inline int32_t geom_grid(const BatchArgs& a) {
   const int64_t blocks = (a.total_tris + 255) / 256;
-  const int64_t cap = (int64_t)a.num_cus * 8;
-  return (int32_t)(blocks < cap ? blocks : cap);
+  int64_t g = a.geom_blocks;
+  if (a.total_tris > g * 4096) {
+    g = (a.total_tris + 1023) / 1024;
+    if (g > kStatSlots) g = kStatSlots;
+  }
+  return (int32_t)(blocks < g ? blocks : g);
 }
 __host__ __device__ inline int32_t bin_grid(const BatchArgs& a) {
   int64_t blocks = (a.nregions + 3) / 4;
@@ -82,6 +92,8 @@ hipError_t launch_geometry(const BatchArgs& a, hipStream_t s);
 hipError_t launch_scan(const BatchArgs& a, hipStream_t s);
 hipError_t launch_bin(const BatchArgs& a, hipStream_t s);
 hipError_t launch_tile(const BatchArgs& a, hipStream_t s);
+// Geometry workgroups one CU holds at once (from the kernel's resources).
+int geom_blocks_per_cu();
 hipError_t launch_shade(const BatchArgs& a, hipStream_t s);
 bool tile_supported(int tile_w, int tile_h);
 // Raster workgroup sizes built per tile size: 8x8 {64, 128}, 16x16 and
