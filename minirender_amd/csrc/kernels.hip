@@ -1023,8 +1023,10 @@ __device__ void reduce_stats(const BatchArgs& a, int tid) {
 }
 
 // (e): one lane per pixel; a workgroup shades a kShadeW x kShadeH block of
-// one job (each wavefront one kShadeW-pixel row segment).
-__global__ __launch_bounds__(kShadeBlock) void k_shade(BatchArgs a) {
+// one job (each wavefront one kShadeW-pixel row segment).  Held to 7 waves
+// per SIMD (72 VGPRs, a few bytes of spill): the kernel is bound by its
+// chain of dependent loads, so more waves in flight pay off; 8 spill too much.
+__global__ __launch_bounds__(kShadeBlock, 7) void k_shade(BatchArgs a) {
   __shared__ int64_t dcache[kMaxLdsDraws];
   const int tid = threadIdx.x;
   const int b = (int)xcd_remap(blockIdx.x, a.shade_blocks);
