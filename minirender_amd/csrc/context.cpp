@@ -275,7 +275,7 @@ static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins
   if ((rc = dev_alloc(c, s.tile_done, nt, true))) return rc;
   if ((rc = dev_alloc(c, s.parts, parts * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.bins, (size_t)bins_cap * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.ckpt_base, (size_t)std::max<int64_t>(2 * T, 1) * 4, true))) return rc;
+  if ((rc = dev_alloc(c, s.ckpt_base, (size_t)std::max<int64_t>(2 * T, 1) * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.ckpt, ckpt_entries(p.npixels) * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.ckpt_ctr, kCkptSegs * kCtrStride * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.heavy_keys, (size_t)heavy_slots * npt, true))) return rc;
@@ -307,7 +307,7 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.bins = (uint2*)s.bins.p;
   a.bins_cap = s.bins.bytes / 8;
   if (a.bins_cap > 0xFFFFFFF0ull) a.bins_cap = 0xFFFFFFF0ull;
-  a.ckpt_base = (uint32_t*)s.ckpt_base.p;
+  a.ckpt_base = (int2*)s.ckpt_base.p;
   a.ckpt = (float2*)s.ckpt.p;
   a.ckpt_cap = std::min<uint64_t>(s.ckpt.bytes / 8, 0xFFFFFFF0ull);
   a.ckpt_ctr = (unsigned long long*)s.ckpt_ctr.p;

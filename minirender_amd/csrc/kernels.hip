@@ -537,17 +537,18 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
   }
 }
 
-constexpr uint32_t kNoCkpt = 0xFFFFFFFFu;
-
-// Edge checkpoints of the wavefront's items: an item spanning ntx > 0 tile
-// columns after its first gets ntx entries per pixel row (one per column it
-// enters), taken by one atomic per wavefront from the checkpoint segment of
-// the wavefront (wavefronts spread over kCkptSegs segments).  Items without
-// checkpoints (one tile column, or no room left) get kNoCkpt.
+// Edge checkpoints of the wavefront's items: an item spanning ntx > 1 tile
+// columns gets ntx entries per pixel row (one per column, where the row
+// starts in it), taken by one atomic per wavefront from the checkpoint
+// segment of the wavefront (wavefronts spread over kCkptSegs segments).
+// Its base is stored pre-offset by its first row and column, so that a
+// reader needs only the row and the tile column.  Items without checkpoints
+// (one tile column, or no room left) get {0, 0}.
 __device__ inline void ckpt_alloc(const BatchArgs& a, bool act, const BinItem& it, int sw) {
   const int lane = threadIdx.x & 63;
-  const int ntx = act ? (it.bx1 >> sw) - (it.bx0 >> sw) : 0;
-  const u64 need = ntx > 0 ? (u64)ntx * (u64)(it.by1 - it.by0 + 1) : 0ull;
+  const int btx = it.bx0 >> sw;
+  const int ntx = act ? (it.bx1 >> sw) - btx + 1 : 0;
+  const u64 need = ntx > 1 ? (u64)ntx * (u64)(it.by1 - it.by0 + 1) : 0ull;
   u64 P = need;  // inclusive prefix over the wave
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -562,14 +563,15 @@ __device__ inline void ckpt_alloc(const BatchArgs& a, bool act, const BinItem& i
   base = __shfl(base, 63);
   if (act) {
     const u64 first = base + P - need;  // within the segment
-    a.ckpt_base[it.id] = (need && first + need <= seg_cap)
-                             ? (uint32_t)(seg * seg_cap + first) : kNoCkpt;
+    const int64_t off = (int64_t)(seg * seg_cap + first) - (int64_t)it.by0 * ntx - btx;
+    const bool ok = need && first + need <= seg_cap && off >= INT32_MIN && off <= INT32_MAX;
+    a.ckpt_base[it.id] = ok ? make_int2((int)off, ntx) : make_int2(0, 0);
   }
 }
 
 // (c) scatter triangle ids into the bin lists of the tiles they overlap and
 // take their edge checkpoints; also resets the merge buffers of split tiles.
-__global__ __launch_bounds__(256) void k_bin(BatchArgs a) {
+__global__ __launch_bounds__(256, 8) void k_bin(BatchArgs a) {
   const int lane = threadIdx.x & 63;
   const u64 np = (u64)a.tile_w * a.tile_h;
   const u64 nheavy = min(a.counters[kHeavy], (u64)a.heavy_cap);
@@ -730,19 +732,16 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
         const int row = max(s.by0, y0) + (q - row_start[i]);
         const int cx0 = max(s.bx0, x0), cx1 = min(s.bx1, x0 + TW - 1);
         const uint32_t id = setup_id[i];
-        // A row entering a tile column after its first leaves a checkpoint
-        // there; the base is loaded first, so its latency overlaps the walk.
-        uint32_t cb = kNoCkpt, cslot = 0;
-        if (cx0 > s.bx0) {
-          cb = a.ckpt_base[id];
-          const int btx = s.bx0 / TW;
-          cslot = (uint32_t)((row - s.by0) * (s.bx1 / TW - btx) + (x0 / TW - btx - 1));
-        }
+        // A row entering a tile column after its triangle's first leaves a
+        // checkpoint there; the base is loaded first, so its latency overlaps
+        // the walk.
+        int2 cb = make_int2(0, 0);
+        if (cx0 > s.bx0) cb = a.ckpt_base[id];
         float e1, e2;
         edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
                    (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
         advance2(e1, s.n1x, e2, s.n2x, cx0 - s.bx0);
-        if (cb != kNoCkpt) a.ckpt[cb + cslot] = make_float2(e1, e2);
+        if (cb.y) a.ckpt[(int64_t)cb.x + (int64_t)row * cb.y + x0 / TW] = make_float2(e1, e2);
         const int prow = (row - y0) * KS - x0;
         for (int c = cx0; c <= cx1; c++) {
           raster_pixel(keys, prow + c, e1, e2, s, persp, id);
@@ -821,7 +820,11 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   // of memory access instead of four.
   const int* ip = D.idx + 3 * tri;
   const int* in = D.nidx + 3 * tri;
-  const uint32_t cb = a.ckpt_base[id];
+  // The pixel's checkpoint needs only the triangle's checkpoint base, its
+  // row and tile column: it is loaded with the vertex records.
+  const int sw = __ffs(a.tile_w) - 1;
+  const int ctx = c >> sw;
+  const int2 cb = a.ckpt_base[id];
   const int i0 = ip[0], i1 = ip[1], i2 = ip[2];
   // Normal and texcoord indices that share the position index buffer are
   // not loaded again.
@@ -854,6 +857,10 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
     U1 = f2{D.uv[2 * t1], D.uv[2 * t1 + 1]};
     U2 = f2{D.uv[2 * t2], D.uv[2 * t2 + 1]};
   }
+  // (The entry of the triangle's first tile column is read too but not used:
+  // its rows start at the bbox.)
+  float2 st = make_float2(0.0f, 0.0f);
+  if (cb.y) st = a.ckpt[(int64_t)cb.x + (int64_t)row * cb.y + ctx];
   // Eye-space vertices as paintMesh computes them (Renderer.cpp:344-349).
   Vtx v0, v1, v2;
   v0.p = ec[0];
@@ -865,11 +872,7 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   // raster's: from the raster's checkpoint where the row enters the pixel's
   // tile column, or from the bbox's left column.
   float e1, e2;
-  const int sw = __ffs(a.tile_w) - 1;
-  const int btx = s.bx0 >> sw, ctx = c >> sw;
-  if (cb != kNoCkpt && ctx > btx) {
-    const float2 st =
-        a.ckpt[cb + (uint32_t)((row - s.by0) * ((s.bx1 >> sw) - btx) + (ctx - btx - 1))];
+  if (cb.y && ctx > (s.bx0 >> sw)) {
     e1 = st.x;
     e2 = st.y;
     advance2(e1, s.n1x, e2, s.n2x, c - (ctx << sw));

@@ -21,9 +21,13 @@ struct BatchArgs {
   uint32_t parts_cap;
   uint2* bins;               // [bins_cap] (triangle id, draw) grouped by tile
   uint64_t bins_cap;
-  // Edge checkpoints: the edge values of a triangle row where it enters each
-  // tile column after its first (written by the raster, read by shading).
-  uint32_t* ckpt_base;       // [2 * total_tris] first checkpoint of a survivor id
+  // Edge checkpoints: the edge values of each row of a triangle spanning
+  // several tile columns where the row enters each of them after the first
+  // (written by the raster, read by shading).  Checkpoint of (row, tile
+  // column tx) of a survivor id: ckpt[ckpt_base[id].x + row *
+  // ckpt_base[id].y + tx], with .x a signed offset (the entries of the first
+  // column are allocated, never written); .y == 0: no checkpoints.
+  int2* ckpt_base;           // [2 * total_tris]
   float2* ckpt;              // [ckpt_cap], kCkptSegs equal segments
   uint64_t ckpt_cap;
   unsigned long long* ckpt_ctr;  // [kCkptSegs * kCtrStride] segment fill counters
