@@ -65,7 +65,7 @@ def parse():
     p.add_argument("--serve-requests", type=int, default=20000)
     p.add_argument("--serve-rate", type=float, default=2000.0)
     p.add_argument("--serve-batch", type=int, default=64)
-    p.add_argument("--serve-slots", type=int, default=3)
+    p.add_argument("--serve-slots", type=int, default=4)
     p.add_argument("--p99-budget", type=float, default=50.0, help="ms")
     return p.parse_args()
 
@@ -418,7 +418,7 @@ def serving_section(args, torch, mra):
             log(f"serving {where} {name}: {res[name]}")
         out[where] = res
     # Highest offered rate each mode sustains with p99 within the budget
-    # (device framebuffers): doubling from 500 req/s, then three bisections.
+    # (device framebuffers): doubling from 500 req/s, then five bisections.
     def sustains(mode, rate):
         count = int(min(n, max(2000, rate * 1.5)))
         r = trace(mode, arrivals(rate, count), count, "device")
@@ -438,7 +438,7 @@ def serving_section(args, torch, mra):
             best, rate = r, rate * 2
         if best is not None and fail is not None:
             lo, hi = best["rate"], fail
-            for _ in range(3):
+            for _ in range(5):
                 mid = (lo + hi) / 2
                 ok, r = sustains(name, mid)
                 if ok:
