@@ -678,6 +678,13 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
     const int local = tile - J.tile_base;
     const int tyi = local / J.tiles_x, txi = local - tyi * J.tiles_x;
     const int x0 = txi * TW, y0 = tyi * TH;
+    if (cnt == 0) {  // nothing binned: background, no key buffer needed
+      for (int p = tid; p < NP; p += NT) {
+        const int c = x0 + (p % TW), row = y0 + (p / TW);
+        if (c < J.W && row < J.H) J.win[(size_t)row * J.W + c] = kNoWinner;
+      }
+      return;
+    }
     for (int p = tid; p < TH * KS; p += NT) keys[p] = ~0ull;
     const uint32_t off = a.tile_off[tile];
     uint32_t avail = cnt;  // entries actually written (bin-list overflow)
