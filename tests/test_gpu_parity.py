@@ -163,6 +163,25 @@ def test_batch_equals_single(gpu_ctx):
                       normals=fr.save_normals)
 
 
+def test_large_batch_equals_single(gpu_ctx):
+    """150 jobs in one pass (job lookups past one wavefront of jobs, many
+    draws, mixed sizes and tiles): every job bit-identical to rendering it
+    alone; a sample of them against the oracle."""
+    frames = [fuzz_frame(400 + i, 24 + (i * 37) % 120, 16 + (i * 23) % 90) for i in range(150)]
+    kinds = ("rgb8", "depth", "winner")
+    gpu_ctx.set_tile(8, 8)
+    try:
+        batch = gpu_ctx.render_batch(frames, kinds)
+        for i, fr in enumerate(frames):
+            single = gpu_ctx.render(fr, kinds)
+            for k in kinds:
+                assert np.array_equal(batch[i][k].view(np.uint8), single[k].view(np.uint8)), (i, k)
+        for i in (0, 63, 64, 65, 149):
+            assert_parity(batch[i], oracle_py.render(frames[i], kinds), f"large batch frame {i}")
+    finally:
+        gpu_ctx.set_tile(16, 16)
+
+
 def test_empty_and_degenerate(gpu_ctx):
     # No draws: background only.
     fr = FrameSpec(40, 30, scenes.projection_frustum(1.0, 4 / 3, 0.1, 10), background=(0.2, 0.4, 0.6))
