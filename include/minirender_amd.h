@@ -154,7 +154,10 @@ int mr_ctx_set_bin_factor(mr_ctx* ctx, float entries_per_triangle);
  * (sizes, scratch, tile configuration) equal the previous one on the same
  * slot replays the captured uploads + kernels instead of launching them one
  * by one.  Off by default; ignored while profiling or on an external stream.
- * No counterpart in the reference (host-side launch overhead). */
+ * On MI355X with ROCm 7.2 a replayed single frame measured slower than eager
+ * launches (which overlap the GPU's work); it is for callers whose host
+ * thread is the bottleneck.  No counterpart in the reference (host-side
+ * launch overhead). */
 int mr_ctx_set_graphs(mr_ctx* ctx, int32_t on);
 /* Record per-stage HIP event times into mr_stats (adds event overhead). */
 int mr_ctx_set_profiling(mr_ctx* ctx, int32_t on);
