@@ -230,6 +230,12 @@ def plain_section(args, torch, mra, frame, wl_name):
             "frame": {"alg_bytes": fb, "gpu_ms": round(fmed, 5),
                       "achieved": round(fb / (fmed * 1e-3) / 1e9, 1),
                       "frac": round(fb / (fmed * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    # How hard the dominant kernel drives HBM: its measured traffic (from the
+    # committed counter summary) over its launch time.
+    if roof["traffic"] and dms > 0:
+        tgbs = roof["traffic"] / (dms * 1e-3) / 1e9
+        roof["traffic_GB/s"] = round(tgbs, 1)
+        roof["traffic_frac"] = round(tgbs / HBM_PEAK_GBS, 4)
     res = {"ms_per_step": dt * 1e3, "value": frame.ntri / dt / 1e6, "frames_per_s": 1.0 / dt,
            "tile": tile, "autotune": tune, "kernels": kernels, "roofline": roof,
            "records": st.records, "bin_entries": st.bin_entries,
