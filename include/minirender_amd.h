@@ -175,6 +175,18 @@ int mr_mesh_upload(mr_ctx* ctx, const float* positions, int32_t npositions,
                    int32_t nuvs, const int32_t* indices,
                    const int32_t* normal_indices, const int32_t* uv_indices,
                    int32_t ntriangles, int32_t* handle);
+/* Upload a TriMesh that uses the vertices, normals and texcoords of the
+ * resident mesh `base`, as the per-material meshes of loadOBJ share one set
+ * of attribute arrays (io.cpp:313-315).  Only the three index arrays are
+ * uploaded (checked against base's attribute counts); uv_indices is ignored
+ * when base has no texcoords.  Draws of meshes sharing attributes with the
+ * same modelview in one frame share one set of vertex records, so the
+ * vertex stage transforms each vertex once per frame instead of once per
+ * mesh.  Releasing base keeps its attribute arrays until every mesh that
+ * shares them is released. */
+int mr_mesh_upload_shared(mr_ctx* ctx, int32_t base, const int32_t* indices,
+                          const int32_t* normal_indices, const int32_t* uv_indices,
+                          int32_t ntriangles, int32_t* handle);
 int mr_mesh_release(mr_ctx* ctx, int32_t handle);
 
 /* Upload a texture (Material::texture, Scene.h:42), rows x cols texels. */

@@ -8,6 +8,7 @@ meshes and textures resident on the GPU and renders FrameSpecs
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
 from typing import Dict, Iterable, List, Optional, Sequence
 
 import numpy as np
@@ -25,6 +26,23 @@ OUTPUT_KINDS = ("rgb8", "depth", "winner", "rgb", "normals", "points")
 
 def _ptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _attr_key(m: Mesh):
+    """Identity of a mesh's attribute arrays: the memory they live in."""
+    def one(a):
+        return None if a is None else (a.__array_interface__["data"][0], a.shape, a.dtype.str)
+    return one(m.positions), one(m.normals), one(m.uvs if m.has_uv else None)
+
+
+def _attr_digest(m: Mesh) -> bytes:
+    """Digest of a mesh's attribute contents (at the time of the call)."""
+    h = hashlib.blake2b(digest_size=16)
+    h.update(m.positions)
+    h.update(m.normals)
+    if m.has_uv:
+        h.update(m.uvs)
+    return h.digest()
 
 
 def _f(seq, n):
@@ -89,6 +107,8 @@ class Context:
         self.h = h
         self._meshes: Dict[int, tuple] = {}
         self._textures: Dict[int, tuple] = {}
+        # attribute-array identity -> (handle of a resident mesh holding them, digest)
+        self._attrs: Dict[tuple, tuple] = {}
 
     def close(self) -> None:
         if self.h:
@@ -148,16 +168,29 @@ class Context:
         uvs = m.uvs if m.has_uv else None
         tix = m.uv_indices if m.has_uv else None
         nix = m.nidx
-        check(self.lib.mr_mesh_upload(
-            self.h, _ptr(m.positions), len(m.positions), _ptr(m.normals), len(m.normals),
-            _ptr(uvs), 0 if uvs is None else len(uvs), _ptr(m.indices), _ptr(nix), _ptr(tix),
-            m.ntri, C.byref(h)), "mr_mesh_upload")
+        # Meshes viewing the same attribute arrays (the per-material parts of
+        # one model) share one resident copy of them, and with it one vertex
+        # transform per frame; the digest guards against arrays changed in
+        # place since the first upload.
+        key, digest = _attr_key(m), _attr_digest(m)
+        base = self._attrs.get(key)
+        if base is not None and base[1] == digest:
+            check(self.lib.mr_mesh_upload_shared(
+                self.h, base[0], _ptr(m.indices), _ptr(nix), _ptr(tix), m.ntri, C.byref(h)),
+                "mr_mesh_upload_shared")
+        else:
+            check(self.lib.mr_mesh_upload(
+                self.h, _ptr(m.positions), len(m.positions), _ptr(m.normals), len(m.normals),
+                _ptr(uvs), 0 if uvs is None else len(uvs), _ptr(m.indices), _ptr(nix),
+                _ptr(tix), m.ntri, C.byref(h)), "mr_mesh_upload")
+            self._attrs[key] = (h.value, digest)
         self._meshes[id(m)] = (h.value, m)
         return h.value
 
     def release_mesh(self, m: Mesh) -> None:
         ent = self._meshes.pop(id(m), None)
         if ent is not None:
+            self._attrs = {k: v for k, v in self._attrs.items() if v[0] != ent[0]}
             check(self.lib.mr_mesh_release(self.h, ent[0]), "mr_mesh_release")
 
     def texture_handle(self, t: Texture) -> int:

@@ -109,6 +109,7 @@ SIGNATURES = {
     "mr_ctx_get_stats": (C.c_int, [_P, C.POINTER(mr_stats)]),
     "mr_ctx_memory": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "mr_mesh_upload": (C.c_int, [_P, _P, _I, _P, _I, _P, _I, _P, _P, _P, _I, _IP]),
+    "mr_mesh_upload_shared": (C.c_int, [_P, _I, _P, _P, _P, _I, _IP]),
     "mr_mesh_release": (C.c_int, [_P, _I]),
     "mr_texture_upload": (C.c_int, [_P, _P, _I, _I, _I, _IP]),
     "mr_texture_release": (C.c_int, [_P, _I]),

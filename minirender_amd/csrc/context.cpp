@@ -173,7 +173,8 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
     J.lighting = f->lighting != 0;
     J.save_normals = f->save_normals != 0;
     uint64_t order = 0;
-    std::map<std::string, int64_t> groups;  // (mesh, modelview) -> first vertex record
+    // (attribute owner, modelview) -> first vertex record
+    std::map<std::string, int64_t> groups;
     for (int d = 0; d < f->ndraws; d++) {
       const mr_draw& s = f->draws[d];
       if (s.mesh < 0 || s.mesh >= (int)c->meshes.size() || !c->meshes[s.mesh].alive)
@@ -182,6 +183,8 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
                              !c->textures[s.texture].alive))
         return MR_ERR_HANDLE;
       const MeshRec& m = c->meshes[s.mesh];
+      const int32_t vowner = m.owner >= 0 ? m.owner : s.mesh;
+      const MeshRec& va = c->meshes[vowner];  // attribute arrays
       DrawDesc D;
       memset(&D, 0, sizeof(D));
       memcpy(D.mv, s.modelview, sizeof(D.mv));
@@ -192,12 +195,12 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
         D.emissive[i] = s.emissive[i];
       }
       D.shininess = s.shininess;
-      D.pos = (const float*)m.pos.p;
-      D.nrm = (const float*)m.nrm.p;
+      D.pos = (const float*)va.pos.p;
+      D.nrm = (const float*)va.nrm.p;
       D.idx = (const int32_t*)m.idx.p;
       D.nidx = (const int32_t*)(m.nidx_alias ? m.idx.p : m.nidx.p);
       if (m.has_uv) {
-        D.uv = (const float*)m.uv.p;
+        D.uv = (const float*)va.uv.p;
         D.tidx = (const int32_t*)(m.tidx_alias ? m.idx.p : m.tidx.p);
       }
       if (f->texturing && s.texture >= 0) {
@@ -214,7 +217,7 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
       D.tile_base = J.tile_base;
       D.tiles_x = J.tiles_x;
       {
-        std::string key((const char*)&s.mesh, sizeof(s.mesh));
+        std::string key((const char*)&vowner, sizeof(vowner));
         key.append((const char*)s.modelview, 12 * sizeof(float));
         auto it = groups.find(key);
         if (it != groups.end()) {
@@ -717,6 +720,20 @@ int mr_ctx_memory(mr_ctx* c, uint64_t* resident, uint64_t* scratch) {
   return MR_OK;
 }
 
+// Store a new mesh record in a free slot (one whose attribute arrays no
+// other mesh still uses); returns its handle.
+static int32_t add_mesh(mr_ctx* c, const MeshRec& m) {
+  MeshRec rec = m;
+  rec.alive = true;
+  for (size_t i = 0; i < c->meshes.size(); i++)
+    if (!c->meshes[i].alive && c->meshes[i].sharers == 0) {
+      c->meshes[i] = rec;
+      return (int32_t)i;
+    }
+  c->meshes.push_back(rec);
+  return (int32_t)c->meshes.size() - 1;
+}
+
 static int upload(mr_ctx* c, DevBuf& b, const void* src, size_t bytes) {
   int rc = dev_alloc(c, b, bytes, false);
   if (rc) return rc;
@@ -761,15 +778,41 @@ int mr_mesh_upload(mr_ctx* c, const float* positions, int32_t npos,
     dev_free(c, m.idx, false); dev_free(c, m.nidx, false); dev_free(c, m.tidx, false);
     return rc;
   }
-  m.alive = true;
-  for (size_t i = 0; i < c->meshes.size(); i++)
-    if (!c->meshes[i].alive) {
-      c->meshes[i] = m;
-      *handle = (int32_t)i;
-      return MR_OK;
-    }
-  c->meshes.push_back(m);
-  *handle = (int32_t)c->meshes.size() - 1;
+  *handle = add_mesh(c, m);
+  return MR_OK;
+}
+
+int mr_mesh_upload_shared(mr_ctx* c, int32_t base, const int32_t* indices,
+                          const int32_t* normal_indices, const int32_t* uv_indices,
+                          int32_t ntri, int32_t* handle) {
+  if (!c || !handle || ntri < 0) return MR_ERR_ARG;
+  if (ntri > 0 && (!indices || !normal_indices)) return MR_ERR_ARG;
+  if (ntri >= (1 << 30)) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (base < 0 || base >= (int)c->meshes.size() || !c->meshes[base].alive) return MR_ERR_HANDLE;
+  const int32_t owner = c->meshes[base].owner >= 0 ? c->meshes[base].owner : base;
+  const MeshRec& o = c->meshes[owner];
+  const bool has_uv = o.has_uv && uv_indices;
+  const int64_t ni = 3 * (int64_t)ntri;
+  if (!indices_ok(indices, ni, o.npos) || !indices_ok(normal_indices, ni, o.nnrm) ||
+      (has_uv && !indices_ok(uv_indices, ni, o.nuv)))
+    return MR_ERR_INDEX;
+  HIP_TRY(hipSetDevice(c->device));
+  MeshRec m;
+  m.npos = o.npos; m.nnrm = o.nnrm; m.nuv = has_uv ? o.nuv : 0; m.ntri = ntri;
+  m.has_uv = has_uv;
+  m.owner = owner;
+  m.nidx_alias = normal_indices == indices;
+  m.tidx_alias = has_uv && uv_indices == indices;
+  int rc;
+  if ((rc = upload(c, m.idx, indices, (size_t)ni * 4)) ||
+      (!m.nidx_alias && (rc = upload(c, m.nidx, normal_indices, (size_t)ni * 4))) ||
+      (has_uv && !m.tidx_alias && (rc = upload(c, m.tidx, uv_indices, (size_t)ni * 4)))) {
+    dev_free(c, m.idx, false); dev_free(c, m.nidx, false); dev_free(c, m.tidx, false);
+    return rc;
+  }
+  c->meshes[owner].sharers++;
+  *handle = add_mesh(c, m);
   return MR_OK;
 }
 
@@ -779,9 +822,15 @@ int mr_mesh_release(mr_ctx* c, int32_t h) {
   if (h < 0 || h >= (int)c->meshes.size() || !c->meshes[h].alive) return MR_ERR_HANDLE;
   hipDeviceSynchronize();  // no batch in flight may still read the mesh
   MeshRec& m = c->meshes[h];
-  dev_free(c, m.pos, false); dev_free(c, m.nrm, false); dev_free(c, m.uv, false);
   dev_free(c, m.idx, false); dev_free(c, m.nidx, false); dev_free(c, m.tidx, false);
   m.alive = false;
+  // Attribute arrays go with the last mesh that uses them.
+  const int32_t owner = m.owner >= 0 ? m.owner : h;
+  if (m.owner >= 0) c->meshes[owner].sharers--;
+  MeshRec& o = c->meshes[owner];
+  if (!o.alive && o.sharers == 0) {
+    dev_free(c, o.pos, false); dev_free(c, o.nrm, false); dev_free(c, o.uv, false);
+  }
   return MR_OK;
 }
 

@@ -26,9 +26,15 @@ struct HostBuf {  // pinned (hipHostMalloc)
 };
 
 // A resident TriMesh (Scene.h:73-87).
+// A resident mesh.  Its attribute arrays (pos, nrm, uv) are its own, or,
+// for a mesh uploaded with mr_mesh_upload_shared, those of mesh `owner`
+// (then its own pos/nrm/uv stay empty).  An owner keeps its attribute
+// arrays, and its slot, after its release until `sharers` drops to zero.
 struct MeshRec {
   DevBuf pos, nrm, uv, idx, nidx, tidx;
   int32_t npos = 0, nnrm = 0, nuv = 0, ntri = 0;
+  int32_t owner = -1;   // mesh holding the attribute arrays, -1: this one
+  int32_t sharers = 0;  // meshes using this mesh's attribute arrays
   bool has_uv = false;
   bool nidx_alias = false;  // normal indices == position indices
   bool tidx_alias = false;  // uv indices == position indices
