@@ -103,6 +103,9 @@ class Renderer {
   std::unordered_map<const TriMesh*, MeshEntry> _meshes;
   std::unordered_map<const Material*, TexEntry> _textures;
   std::unordered_map<int32_t, const TriMesh*> _handleMesh;
+  // meshes of the scene being described, which may lend attribute arrays
+  std::vector<const TriMesh*> _frameMeshes;
+  bool _sharing = false;
   std::vector<mr_draw> _draws;
   mr_frame _frame{};
   // immediate-mode draws since the last render()

@@ -389,6 +389,15 @@ static void hashArray(uint64_t& h, const std::vector<T>& v) {
   }
 }
 
+template <class T>
+static bool sameBytes(const std::vector<T>& a, const std::vector<T>& b) {
+  return a.size() == b.size() && (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(T)) == 0);
+}
+
+static bool usesTexcoords(const TriMesh* m) {
+  return !m->texcoords.empty() && m->texcoordsI.size() == m->indices.size();
+}
+
 int32_t Renderer::meshHandle(TriMesh* m) {
   uint64_t fp = 1469598103934665603ull;
   hashArray(fp, m->vertices);
@@ -398,7 +407,10 @@ int32_t Renderer::meshHandle(TriMesh* m) {
   hashArray(fp, m->normalsI);
   hashArray(fp, m->texcoordsI);
   MeshEntry& e = _meshes[m];
-  if (e.handle >= 0 && e.fingerprint == fp) return e.handle;
+  if (e.handle >= 0 && e.fingerprint == fp) {
+    if (_sharing) _frameMeshes.push_back(m);
+    return e.handle;
+  }
   mr_ctx* c = context();
   if (!c) return -1;
   if (e.handle >= 0) {
@@ -407,18 +419,41 @@ int32_t Renderer::meshHandle(TriMesh* m) {
   }
   const int ntri = (int)(m->indices.size() / 3);
   const bool nI = m->normalsI.size() == m->indices.size();
-  const bool uv = !m->texcoords.empty() && m->texcoordsI.size() == m->indices.size();
+  const bool uv = usesTexcoords(m);
+  // A mesh of the scene being described whose attribute arrays hold the
+  // same bytes (the per-material meshes of one model, io.cpp:313-315)
+  // lends them: one resident copy and one vertex transform per frame.
+  int32_t base = -1;
+  if (_sharing)
+    for (const TriMesh* o : _frameMeshes) {
+      auto it = _meshes.find(o);
+      if (o == m || it == _meshes.end() || it->second.handle < 0) continue;
+      if ((!uv || usesTexcoords(o)) && sameBytes(o->vertices, m->vertices) &&
+          sameBytes(o->normals, m->normals) && sameBytes(o->texcoords, m->texcoords)) {
+        base = it->second.handle;
+        break;
+      }
+    }
   int32_t h = -1;
-  _status = mr_mesh_upload(
-      c, (const float*)m->vertices.data(), (int32_t)m->vertices.size(),
-      (const float*)m->normals.data(), (int32_t)m->normals.size(),
-      uv ? (const float*)m->texcoords.data() : nullptr, uv ? (int32_t)m->texcoords.size() : 0,
-      m->indices.data(), nI ? m->normalsI.data() : m->indices.data(),
-      uv ? m->texcoordsI.data() : nullptr, ntri, &h);
+  if (base >= 0) {
+    _status = mr_mesh_upload_shared(c, base, m->indices.data(),
+                                    nI ? m->normalsI.data() : m->indices.data(),
+                                    uv ? m->texcoordsI.data() : nullptr, ntri, &h);
+  } else {
+    _status = mr_mesh_upload(
+        c, (const float*)m->vertices.data(), (int32_t)m->vertices.size(),
+        (const float*)m->normals.data(), (int32_t)m->normals.size(),
+        uv ? (const float*)m->texcoords.data() : nullptr, uv ? (int32_t)m->texcoords.size() : 0,
+        m->indices.data(), nI ? m->normalsI.data() : m->indices.data(),
+        uv ? m->texcoordsI.data() : nullptr, ntri, &h);
+  }
   e.handle = _status == MR_OK ? h : -1;
   e.fingerprint = fp;
   e.mesh = m;
-  if (e.handle >= 0) _handleMesh[e.handle] = m;
+  if (e.handle >= 0) {
+    _handleMesh[e.handle] = m;
+    if (_sharing) _frameMeshes.push_back(m);
+  }
   return e.handle;
 }
 
@@ -497,10 +532,14 @@ const mr_frame& Renderer::describeFrame() {
     _ambient = _scene->ambientLight;  // :322
     Array<Renderable> list;
     _scene->collectShapes(list, Matrix4::identity());  // :315
+    _sharing = true;  // the scene's meshes stay alive while it is described
+    _frameMeshes.clear();
     for (const Renderable& r : list) {
       const Material& mat = r.mesh->material ? *r.mesh->material : *_defmaterial;  // :336
       appendDraw(r.mesh, r.transform, mat, _draws);
     }
+    _sharing = false;
+    _frameMeshes.clear();
   }
   fillFrame(_frame, _draws);
   return _frame;

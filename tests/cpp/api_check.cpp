@@ -6,7 +6,8 @@
 //   api_check gpu   Renderer::render() on the GPU against the CPU oracle for
 //                   scenes built with the API (nested nodes, transforms,
 //                   materials, a float texture, point light, immediate-mode
-//                   paintMesh after render): winner and depth bit-equal,
+//                   paintMesh after render, a two-material OBJ whose meshes
+//                   share one attribute upload): winner and depth bit-equal,
 //                   RGB8 within 1 LSB.
 // Prints "ok" and exits 0, or prints the first failure and exits 1.
 #include <cmath>
@@ -385,6 +386,71 @@ static int gpu_checks() {
       CHECK(oracle_render(r, f2, all, texs2, depth, win, rgb8) == 0, "oracle immediate");
       if (compare(r, depth, win, rgb8, "immediate")) return 1;
     }
+  }
+  // A two-material OBJ: loadOBJ's meshes share one vertex and normal array
+  // (io.cpp:313-315); the renderer uploads it once and the image matches.
+  {
+    FILE* mt = std::fopen("/tmp/mr_api_check_grid.mtl", "w");
+    std::fprintf(mt, "newmtl a\nKd 0.1 0.6 0.3\nnewmtl b\nKd 0.7 0.4 0.2\nNs 20\n");
+    std::fclose(mt);
+    FILE* o = std::fopen("/tmp/mr_api_check_grid.obj", "w");
+    std::fprintf(o, "mtllib mr_api_check_grid.mtl\nvn 0 0 1\n");
+    const int n = 9;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++)
+        std::fprintf(o, "v %g %g %g\n", -1.0 + 2.0 * j / (n - 1), -1.0 + 2.0 * i / (n - 1),
+                     0.15 * ((i * 7 + j * 3) % 5));
+    for (int i = 0; i + 1 < n; i++) {
+      std::fprintf(o, "usemtl %s\n", i < n / 2 ? "a" : "b");
+      for (int j = 0; j + 1 < n; j++) {
+        const int v = i * n + j + 1;
+        std::fprintf(o, "f %d//1 %d//1 %d//1 %d//1\n", v, v + 1, v + n + 1, v + n);
+      }
+    }
+    std::fclose(o);
+    Shared<SceneNode> grid = loadOBJ("/tmp/mr_api_check_grid.obj");
+    // meshes "" (no faces), "a", "b", each with all 81 vertices
+    CHECK(grid && grid->children.size() == 3, "grid OBJ meshes");
+    auto* g0 = dynamic_cast<TriMesh*>(grid->children[1].get());
+    auto* g1 = dynamic_cast<TriMesh*>(grid->children[2].get());
+    CHECK(g0 && g1 && g0->vertices.size() == (size_t)(n * n) &&
+              g1->vertices.size() == g0->vertices.size(), "grid OBJ vertices");
+    auto gscene = std::make_shared<Scene>();
+    gscene->add(grid);
+    auto one = std::make_shared<Scene>();
+    one->add(grid->children[0]);
+    one->add(grid->children[1]);
+    Renderer rg, r0;
+    for (Renderer* q : {&rg, &r0}) {
+      q->setLight(Vec3(1, 2, 3), true);
+      q->setSize(160, 120);
+      q->setProjection(projectionFrustum(0.9f, q->aspect(), 0.1f, 50.0f));
+      q->setView(Matrix4::translate(0, 0, -3.5f) * Matrix4::rotateX(-0.4f));
+    }
+    rg.setScene(gscene);
+    r0.setScene(one);
+    rg.render();
+    r0.render();
+    CHECK(rg.lastStatus() == 0 && r0.lastStatus() == 0, "grid render");
+    uint64_t res_g = 0, res_0 = 0, scr = 0;
+    CHECK(mr_ctx_memory(rg.context(), &res_g, &scr) == 0 &&
+              mr_ctx_memory(r0.context(), &res_0, &scr) == 0, "grid memory");
+    // Mesh "b" adds only its index arrays (positions and normal indices;
+    // device allocations are rounded up to 256 bytes).
+    const uint64_t ibytes = (g1->indices.size() * sizeof(int) + 255) & ~(uint64_t)255;
+    CHECK(res_g - res_0 == 2 * ibytes, "grid arrays shared: %llu %llu",
+          (unsigned long long)res_g, (unsigned long long)res_0);
+    const mr_frame f = rg.describeFrame();
+    std::vector<mr_draw> draws(f.draws, f.draws + f.ndraws);
+    std::vector<const Array2<Vec3>*> texs(draws.size(), nullptr);
+    std::vector<float> depth;
+    std::vector<uint32_t> win;
+    std::vector<uint8_t> rgb8;
+    CHECK(oracle_render(rg, f, draws, texs, depth, win, rgb8) == 0, "grid oracle");
+    if (compare(rg, depth, win, rgb8, "grid")) return 1;
+    size_t covered = 0;
+    for (uint32_t w : win) covered += w != 0xFFFFFFFFu;
+    CHECK(covered > 4000, "grid coverage %zu", covered);
   }
   std::printf("ok\n");
   return 0;
