@@ -67,6 +67,8 @@ def parse():
     p.add_argument("--serve-batch", type=int, default=64)
     p.add_argument("--serve-slots", type=int, default=4)
     p.add_argument("--p99-budget", type=float, default=50.0, help="ms")
+    p.add_argument("--serve-batch-profile", action="store_true",
+                   help="only time one batch of --serve-batch serving requests (GPU stages)")
     return p.parse_args()
 
 
@@ -94,15 +96,22 @@ def input_bytes(frame, with_attrs=True):
     attributes also normals 12 B, uvs 8 B (when used) and their index arrays
     when they are distinct arrays."""
     b = 0
+    seen = set()  # attribute arrays shared by several draws are read once
     for d in frame.draws:
         m = d.mesh
-        b += 12 * len(m.positions) + 12 * m.ntri
+        b += 12 * m.ntri
+        shared = m.positions.__array_interface__["data"][0] in seen
+        seen.add(m.positions.__array_interface__["data"][0])
+        if not shared:
+            b += 12 * len(m.positions)
         if with_attrs:
-            b += 12 * len(m.normals)
+            if not shared:
+                b += 12 * len(m.normals)
             if m.normal_indices is not None and m.normal_indices is not m.indices:
                 b += 12 * m.ntri
             if m.has_uv and frame.texturing and d.material.texture is not None:
-                b += 8 * len(m.uvs)
+                if not shared:
+                    b += 8 * len(m.uvs)
                 if m.uv_indices is not m.indices:
                     b += 12 * m.ntri
     return b
@@ -347,6 +356,59 @@ def batched_section(args, torch, mra):
             "speedup": tp / tb}
 
 
+def serve_batch_profile(args, torch, mra):
+    """One batch of the service's requests (--serve-batch of them over the
+    resident pool, device outputs), rendered --steps times with the batch's
+    autotuned launch configuration: wall time per batch and the GPU time of
+    each stage (HIP events, in a separate profiled pass)."""
+    from minirender_amd import scenes
+    from minirender_amd._capi import mr_frame, mr_outputs
+    pool = scenes.serving_pool()
+    reqs = scenes.serving_requests(pool, args.serve_batch)
+    ctx = mra.Context(0)
+    dev = torch.device("cuda:0")
+    structs, keep, outs, bufs = [], [], [], []
+    for fr in reqs:
+        s, k = ctx.frame(fr)
+        o, b = device_outputs(fr, torch, dev)
+        structs.append(s)
+        keep.append(k)
+        outs.append(o)
+        bufs.append(b)
+    n = len(reqs)
+    farr = (mr_frame * n)(*structs)
+    oarr = (mr_outputs * n)(*outs)
+    tune = ctx.autotune(reqs, outs=outs, reps=2, cache_path=args.tune_cache)
+
+    def batch():
+        rc = ctx.lib.mr_render_batch(ctx.h, farr, n, oarr)
+        if rc:
+            raise RuntimeError(f"mr_render_batch failed: {rc}")
+
+    for _ in range(args.warmup):
+        batch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    ctx.set_profiling(True)
+    st, fm = [], []
+    for _ in range(max(3, args.steps // 3)):
+        batch()
+        s = ctx.stats()
+        st.append(list(s.stage_ms[:len(STAGES)]))
+        fm.append(s.frame_ms)
+    med = np.median(np.array(st), 0)
+    ctx.close()
+    return {"serve_batch": n, "triangles": sum(r.ntri for r in reqs),
+            "pixels": sum(r.width * r.height for r in reqs), "autotune": tune,
+            "ms_per_batch": round(ms, 4), "requests_per_s": round(n / ms * 1e3, 1),
+            "gpu_ms": round(float(np.median(fm)), 4),
+            "stage_ms": {k: round(float(v), 4) for k, v in zip(STAGES, med)}}
+
+
 def serving_section(args, torch, mra):
     from minirender_amd import scenes
     from minirender_amd._capi import mr_frame, mr_outputs
@@ -476,6 +538,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n, "copies": 5}), flush=True)
+        return
+    if args.serve_batch_profile:
+        import torch
+        import minirender_amd as mra
+        print(json.dumps(serve_batch_profile(args, torch, mra)), flush=True)
         return
     frame, wl = make_frame(args.workload)
     log(f"workload {args.workload}: {frame.ntri} triangles")
