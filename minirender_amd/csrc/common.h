@@ -92,6 +92,7 @@ enum Counter : int {
   kParts = 7,        // work items of split tiles (written by the scan)
   kHeavy = 8,        // tiles split over several work items
   kHeavyOverflow = 9,
+  kListed = 10,      // tiles in the raster's tile list (written by the scan)
   kNumCounters = 16
 };
 
@@ -128,6 +129,7 @@ constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;    // elements per thread in the tile-count scan
 constexpr uint32_t kNoWinner = 0xFFFFFFFFu;
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr uint32_t kEmptyTile = 0xFFFFFFFEu;  // tile_heavy of a tile with no bin entry
 constexpr float kClearDepth = 1e11f;  // Renderer.cpp:116
 constexpr int kMaxDim = 16384;        // max width / height
 

@@ -25,8 +25,8 @@ struct HostBuf {  // pinned (hipHostMalloc)
   size_t bytes = 0;
 };
 
-// A resident TriMesh (Scene.h:73-87).
-// A resident mesh.  Its attribute arrays (pos, nrm, uv) are its own, or,
+// A resident TriMesh (Scene.h:73-87).  Its attribute arrays (pos, nrm, uv)
+// are its own, or,
 // for a mesh uploaded with mr_mesh_upload_shared, those of mesh `owner`
 // (then its own pos/nrm/uv stay empty).  An owner keeps its attribute
 // arrays, and its slot, after its release until `sharers` drops to zero.
@@ -49,7 +49,7 @@ struct TexRec {
 
 // Device scratch of one batch in flight, and its pinned staging.
 struct Scratch {
-  DevBuf items, region_count, tile_count, tile_off, tile_cursor, tile_heavy,
+  DevBuf items, region_count, tile_count, tile_off, tile_cursor, tile_heavy, tile_list,
       tile_done, parts, bins, ckpt_base, ckpt, ckpt_ctr, heavy_keys, scan_status, counters,
       block_stats, tables, vrec, win, outputs;
   // The batch tables (jobs, job tile/shade bases, draws, draw starts) sit in

@@ -11,18 +11,19 @@
 //             (ballot + popcount prefix, no global atomic) and counts its
 //             (tile, triangle) pairs with one atomic per distinct tile
 //   k_scan    (c) exclusive scan of the tile counts (decoupled look-back);
-//             tiles with more pairs than one work item are split into parts
+//             tiles with more pairs than one work item are split into parts,
+//             the others with pairs listed for the raster, empty ones marked
 //   k_bin     (c) scatter of (triangle id, draw) into the tiles' bin lists:
 //             per wavefront step, one atomic instruction for all distinct
 //             tiles; takes the edge checkpoints of triangles spanning
 //             several tile columns
-//   k_tile    (d) per-tile raster with the tile's 64-bit
+//   k_tile    (d) per-tile raster (listed tiles and parts) with the tile's 64-bit
 //             (depth key, draw-order id) buffer in LDS and ds_min_u64; parts
 //             of a split tile merge through global 64-bit atomicMin.  Writes
 //             each pixel's winning id and the rows' edge checkpoints
 //   k_shade   (e) deferred Blinn-Phong + texture, one lane per pixel in image
 //             order, writing RGB8 / depth / winner / float colour / normals /
-//             range image
+//             range image (background for the pixels of empty tiles)
 //
 // A triangle's id is 2 * (flat triangle index in the batch) + clip sub-index,
 // which increases with draw order inside a job.  The minimum of
@@ -239,42 +240,69 @@ __device__ inline void tri_setup_sub(const BatchArgs& a, const DrawDesc& D, cons
   tri_setup_corners(J, e, v, sub, s);
 }
 
-// (a): one lane per vertex of every vertex group: eye-space position and
-// its projection (Renderer.cpp:340-349, 191-203), once per vertex instead of
-// once per triangle corner in each later kernel.
+// (a): kVertsPerLane vertices per lane of every vertex group: eye-space
+// position and its projection (Renderer.cpp:340-349, 191-203), once per
+// vertex instead of once per triangle corner in each later kernel.  A
+// wavefront takes 64 * kVertsPerLane consecutive vertices per step and loads
+// all of a lane's positions before transforming any: the kernel is bound by
+// loads in flight.
+constexpr int kVertsPerLane = 2;
+constexpr int kVertexStep = 256 * kVertsPerLane;  // vertices per workgroup step
+
+__device__ inline void vertex_record(const BatchArgs& a, const DrawDesc& D, const JobDesc& J,
+                                     int64_t i, f3 p) {
+  const f3 e = xform(D.mv, p);
+  const ScreenVtx v = project_vertex(J.P, J.persp != 0, (float)J.W, (float)J.H, e);
+  float2* r = reinterpret_cast<float2*>(a.vrec + i * kVrecFloats);
+  r[0] = make_float2(e.x, e.y);
+  r[1] = make_float2(e.z, v.px);
+  r[2] = make_float2(v.py, v.zv);
+}
+
 __global__ __launch_bounds__(256) void k_vertex(BatchArgs a) {
-  // Contiguous range per workgroup: one group search, then stepping.
-  const int64_t chunk = ((a.total_verts + (int64_t)gridDim.x * 256 - 1) /
-                         ((int64_t)gridDim.x * 256)) * 256;
+  // Contiguous range per workgroup (a multiple of kVertexStep): one group
+  // search, then stepping.
+  const int64_t chunk = ((a.total_verts + (int64_t)gridDim.x * kVertexStep - 1) /
+                         ((int64_t)gridDim.x * kVertexStep)) * kVertexStep;
   const int64_t r_begin = (int64_t)blockIdx.x * chunk;
   const int64_t r_end = min(r_begin + chunk, a.total_verts);
+  const int lane = threadIdx.x & 63;
   int gw = search_last(a.vg_begin, 0, a.nvgroups - 1,
                        (int64_t)__builtin_amdgcn_readfirstlane((int)min(r_begin, a.total_verts)));
   gw = __builtin_amdgcn_readfirstlane(gw);
-  for (int64_t base = r_begin; base < r_end; base += 256) {
-    const int64_t wfirst = base + (threadIdx.x & ~63);
+  for (int64_t base = r_begin; base < r_end; base += kVertexStep) {
+    const int64_t wfirst = base + kVertsPerLane * (threadIdx.x & ~63);
     if (wfirst >= r_end) continue;
-    const int64_t i = wfirst + (threadIdx.x & 63);
     while (gw + 1 < a.nvgroups && a.vg_begin[gw + 1] <= wfirst) gw++;
     gw = __builtin_amdgcn_readfirstlane(gw);
-    if (i >= r_end) continue;
     const int64_t gw_end = gw + 1 < a.nvgroups ? a.vg_begin[gw + 1] : 0x7FFFFFFFFFFFFFFFll;
-    auto emit = [&](const DrawDesc& D, const JobDesc& J, int64_t gbegin) {
-      const f3 e = xform(D.mv, load3(D.pos, (int)(i - gbegin)));
-      const ScreenVtx v = project_vertex(J.P, J.persp != 0, (float)J.W, (float)J.H, e);
-      float2* r = reinterpret_cast<float2*>(a.vrec + i * kVrecFloats);
-      r[0] = make_float2(e.x, e.y);
-      r[1] = make_float2(e.z, v.px);
-      r[2] = make_float2(v.py, v.zv);
-    };
-    if (__ballot(i < gw_end) == __ballot(1)) {  // one group: descriptors via scalar loads
+    if (min(wfirst + 64 * kVertsPerLane - 1, r_end - 1) < gw_end) {
+      // The wavefront's vertices are all in group gw: descriptors through
+      // scalar loads.
       const DrawDesc Du = uniform_draw(a, __builtin_amdgcn_readfirstlane(a.vg_draw[gw]));
-      emit(Du, uniform_job(a, __builtin_amdgcn_readfirstlane(Du.job)), a.vg_begin[gw]);
+      const JobDesc Ju = uniform_job(a, __builtin_amdgcn_readfirstlane(Du.job));
+      const int64_t gb = a.vg_begin[gw];
+      f3 p[kVertsPerLane];
+#pragma unroll
+      for (int h = 0; h < kVertsPerLane; h++) {
+        const int64_t i = wfirst + 64 * h + lane;
+        p[h] = f3{0.0f, 0.0f, 0.0f};
+        if (i < r_end) p[h] = load3(Du.pos, (int)(i - gb));
+      }
+#pragma unroll
+      for (int h = 0; h < kVertsPerLane; h++) {
+        const int64_t i = wfirst + 64 * h + lane;
+        if (i < r_end) vertex_record(a, Du, Ju, i, p[h]);
+      }
     } else {
-      int g = gw;
-      while (g + 1 < a.nvgroups && a.vg_begin[g + 1] <= i) g++;
-      const DrawDesc& D = a.draws[a.vg_draw[g]];
-      emit(D, a.jobs[D.job], a.vg_begin[g]);
+      for (int h = 0; h < kVertsPerLane; h++) {
+        const int64_t i = wfirst + 64 * h + lane;
+        if (i >= r_end) continue;
+        int g = gw;
+        while (g + 1 < a.nvgroups && a.vg_begin[g + 1] <= i) g++;
+        const DrawDesc& D = a.draws[a.vg_draw[g]];
+        vertex_record(a, D, a.jobs[D.job], i, load3(D.pos, (int)(i - a.vg_begin[g])));
+      }
     }
   }
 }
@@ -471,46 +499,56 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
     if (blk == a.scan_blocks - 1) a.counters[kBinEntries] = excl + btot;
   }
   // Split (heavy) tiles: the workgroup reserves its merge slots and part
-  // list entries with one atomic each (same-address atomics serialise).
-  uint32_t nh = 0, npt = 0;
+  // list entries with one atomic each (same-address atomics serialise); the
+  // same for its entries of the raster's tile list (tiles rastered whole:
+  // with entries, not split).
+  uint32_t nh = 0, npt = 0, nl = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++)
     if (base + i < a.ntiles && v[i] > a.frag_size) {
       nh++;
       npt += (v[i] + a.frag_size - 1) / a.frag_size;
+    } else if (base + i < a.ntiles && v[i] > 0) {
+      nl++;
     }
-  uint32_t hx = nh, px = npt;  // wave inclusive scans
+  uint32_t hx = nh, px = npt, lx = nl;  // wave inclusive scans
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t yh = __shfl_up(hx, o), yp = __shfl_up(px, o);
+    const uint32_t yh = __shfl_up(hx, o), yp = __shfl_up(px, o), yl = __shfl_up(lx, o);
     if (lane >= o) {
       hx += yh;
       px += yp;
+      lx += yl;
     }
   }
-  __shared__ uint32_t sh_h[kScanBlock / 64], sh_p[kScanBlock / 64];
-  __shared__ u64 sh_hbase, sh_pbase;
+  __shared__ uint32_t sh_h[kScanBlock / 64], sh_p[kScanBlock / 64], sh_l[kScanBlock / 64];
+  __shared__ u64 sh_hbase, sh_pbase, sh_lbase;
   if (lane == 63) {
     sh_h[wave] = hx;
     sh_p[wave] = px;
+    sh_l[wave] = lx;
   }
   __syncthreads();
-  uint32_t hpre = 0, ppre = 0, htot = 0, ptot = 0;
+  uint32_t hpre = 0, ppre = 0, lpre = 0, htot = 0, ptot = 0, ltot = 0;
 #pragma unroll
   for (int w = 0; w < kScanBlock / 64; w++) {
     if (w < wave) {
       hpre += sh_h[w];
       ppre += sh_p[w];
+      lpre += sh_l[w];
     }
     htot += sh_h[w];
     ptot += sh_p[w];
+    ltot += sh_l[w];
   }
   if (tid == 0) {
     sh_hbase = htot ? atomicAdd(&a.counters[kHeavy], (u64)htot) : 0;
     sh_pbase = ptot ? atomicAdd(&a.counters[kParts], (u64)ptot) : 0;
+    sh_lbase = ltot ? atomicAdd(&a.counters[kListed], (u64)ltot) : 0;
   }
   __syncthreads();
   u64 hs = sh_hbase + hpre + (hx - nh), ps = sh_pbase + ppre + (px - npt);
+  u64 ls = sh_lbase + lpre + (lx - nl);
   u64 run = sh_excl + wpre + (x - sum);
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
@@ -530,6 +568,10 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
         }
         hs++;
         ps += nparts;
+      } else if (v[i] > 0) {
+        a.tile_list[ls++] = (uint32_t)t;
+      } else {
+        slot = kEmptyTile;  // shading writes its pixels' background
       }
       a.tile_heavy[t] = slot;
     }
@@ -663,9 +705,12 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
     tile = (int)pp.x;
     part = (int)pp.y;
   } else {
+    // Tiles rastered whole come from the scan's list; tiles without bin
+    // entries are not in it (shading writes their background).
     const int64_t t = b - nparts_all;
-    if (t >= a.ntiles) return;
-    tile = (int)xcd_remap(t, a.ntiles);
+    const int64_t nlisted = (int64_t)min(a.counters[kListed], (u64)a.ntiles);
+    if (t >= nlisted) return;
+    tile = (int)a.tile_list[xcd_remap(t, nlisted)];
     part = 0;
   }
   {
@@ -678,13 +723,6 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
     const int local = tile - J.tile_base;
     const int tyi = local / J.tiles_x, txi = local - tyi * J.tiles_x;
     const int x0 = txi * TW, y0 = tyi * TH;
-    if (cnt == 0) {  // nothing binned: background, no key buffer needed
-      for (int p = tid; p < NP; p += NT) {
-        const int c = x0 + (p % TW), row = y0 + (p / TW);
-        if (c < J.W && row < J.H) J.win[(size_t)row * J.W + c] = kNoWinner;
-      }
-      return;
-    }
     for (int p = tid; p < TH * KS; p += NT) keys[p] = ~0ull;
     const uint32_t off = a.tile_off[tile];
     uint32_t avail = cnt;  // entries actually written (bin-list overflow)
@@ -1049,9 +1087,16 @@ __global__ __launch_bounds__(kShadeBlock, 7) void k_shade(BatchArgs a) {
   const int c = (local - byi * bxn) * kShadeW + (tid - r * kShadeW), row = byi * kShadeH + r;
   const bool inside = c < J.W && row < J.H;
   // The winner load is issued first, so that its latency overlaps the
-  // draw-table fill below.
+  // draw-table fill below.  A tile without bin entries has no winners
+  // written (the raster skips it): its pixels are background, and their
+  // winners are not read.
   const size_t pix = (size_t)row * J.W + c;
-  const uint32_t id = inside ? J.win[pix] : kNoWinner;
+  uint32_t id = kNoWinner;
+  if (inside) {
+    const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
+    const uint32_t flag = a.tile_heavy[J.tile_base + (row >> sh) * J.tiles_x + (c >> sw)];
+    if (flag != kEmptyTile) id = J.win[pix];
+  }
   // A job of one draw needs no draw table (ncached = -1).
   const bool one_draw = J.draw_end - J.draw_begin == 1;
   const int ncached = one_draw ? -1 : fill_draw_cache(dcache, a, J, tid, kShadeBlock);
@@ -1093,7 +1138,7 @@ int geom_blocks_per_cu() {
 
 hipError_t launch_vertex(const BatchArgs& a, hipStream_t s) {
   if (a.total_verts == 0) return hipSuccess;
-  const int64_t blocks = (a.total_verts + 255) / 256;
+  const int64_t blocks = (a.total_verts + kVertexStep - 1) / kVertexStep;
   const int grid = (int)std::min<int64_t>(blocks, (int64_t)a.num_cus * 8);
   hipLaunchKernelGGL(k_vertex, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
