@@ -131,6 +131,9 @@ constexpr uint32_t kNoWinner = 0xFFFFFFFFu;
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 constexpr uint32_t kEmptyTile = 0xFFFFFFFEu;  // tile_heavy of a tile with no bin entry
 constexpr float kClearDepth = 1e11f;  // Renderer.cpp:116
+// Job lookup hints: one per kHintSpan consecutive tiles or shading blocks.
+constexpr int kHintShift = 6;
+constexpr int kHintSpan = 1 << kHintShift;
 constexpr int kMaxDim = 16384;        // max width / height
 
 }  // namespace mr

@@ -241,6 +241,19 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
     J.draw_end = (int32_t)plan.draws.size();
   }
   if (2 * plan.total_tris >= 0xFFFFFFFFll || plan.total_verts >= 0x7FFFFFFFll) return MR_ERR_ARG;
+  // Job hints for the raster's and shading's job lookup: every kHintSpan-th
+  // tile (block), its job and the next job's first tile (block).
+  plan.hints.clear();
+  for (int k = 0; k < 2; k++) {
+    const int32_t* base = plan.job_bases.data() + (size_t)k * nframes;
+    const int64_t n = k == 0 ? plan.ntiles : plan.shade_blocks;
+    int j = 0;
+    for (int64_t x = 0; x < n; x += kHintSpan) {
+      while (j + 1 < nframes && base[j + 1] <= x) j++;
+      plan.hints.push_back(make_int2(j, j + 1 < nframes ? base[j + 1] : 0x7FFFFFFF));
+    }
+    if (k == 0) plan.tile_hints = plan.hints.size();
+  }
   auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
   TableLayout& L = plan.layout;
   const size_t nd = std::max<size_t>(plan.draws.size(), 1);
@@ -251,7 +264,8 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
   const size_t ng = std::max<size_t>(plan.vg_begin.size(), 1);
   L.vg_begin = L.draw_begin + up(nd * 8);
   L.vg_draw = L.vg_begin + up(ng * 8);
-  L.bytes = L.vg_draw + up(ng * 4);
+  L.hints = L.vg_draw + up(ng * 4);
+  L.bytes = L.hints + up(std::max<size_t>(plan.hints.size(), 1) * 8);
   return MR_OK;
 }
 
@@ -334,6 +348,8 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.jobs = (const JobDesc*)(tb + p.layout.jobs);
   a.job_tile_base = (const int32_t*)(tb + p.layout.bases);
   a.job_shade_base = (const int32_t*)(tb + p.layout.bases) + p.jobs.size();
+  a.tile_hint = (const int2*)(tb + p.layout.hints);
+  a.shade_hint = a.tile_hint + p.tile_hints;
   a.njobs = (int32_t)p.jobs.size();
   a.total_tris = p.total_tris;
   a.ntiles = (int32_t)p.ntiles;
@@ -396,6 +412,7 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
     memcpy(ht + p.layout.vg_begin, p.vg_begin.data(), p.vg_begin.size() * 8);
     memcpy(ht + p.layout.vg_draw, p.vg_draw.data(), p.vg_draw.size() * 4);
   }
+  memcpy(ht + p.layout.hints, p.hints.data(), p.hints.size() * 8);
   const BatchArgs a = make_args(c, s, p);
   s.last_geom_grid = p.total_tris ? geom_grid(a) : 0;
   s.last_bin_grid = bin_grid(a);

@@ -59,7 +59,8 @@ struct Scratch {
 
 // Byte offsets of the tables inside Scratch::tables.
 struct TableLayout {
-  size_t jobs = 0, bases = 0, draws = 0, draw_begin = 0, vg_begin = 0, vg_draw = 0, bytes = 0;
+  size_t jobs = 0, bases = 0, draws = 0, draw_begin = 0, vg_begin = 0, vg_draw = 0, hints = 0,
+         bytes = 0;
 };
 
 // Jobs and draws of a batch, as the kernels see them.
@@ -68,6 +69,8 @@ struct BatchPlan {
   std::vector<DrawDesc> draws;
   std::vector<int64_t> draw_begin;
   std::vector<int32_t> job_bases;  // tile bases of all jobs, then shade bases
+  std::vector<int2> hints;          // job hints of the tiles, then of the shading blocks
+  size_t tile_hints = 0;
   // Vertex groups: draws of one job with the same mesh and modelview share
   // their projected vertices.
   std::vector<int64_t> vg_begin;

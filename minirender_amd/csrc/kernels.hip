@@ -107,6 +107,21 @@ __device__ inline int search_last(const T* begin, int lo, int hi, int64_t x) {
   return lo;
 }
 
+// Job of tile (shading block) x from the job hints: the job of x's span of
+// kHintSpan, stepped past job boundaries inside the span (at most one unless
+// jobs are smaller than the span).  One scalar load where a binary search
+// over the jobs' first tiles takes log2(njobs) dependent ones.
+__device__ inline int job_lookup(const int2* hint, const int32_t* base, int njobs, int64_t x) {
+  if (njobs == 1) return 0;
+  const int2 h = hint[x >> kHintShift];
+  int j = h.x;
+  if (x >= h.y) {
+    j++;
+    while (j + 1 < njobs && (int64_t)base[j + 1] <= x) j++;
+  }
+  return j;
+}
+
 // Draw of flat triangle g in job J: the job's only draw (ncached < 0), the
 // job's draw table cached in LDS when it fits (ncached > 0), else searched in
 // global memory.
@@ -717,7 +732,7 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
     const uint32_t cnt = a.tile_count[(size_t)tile * kCtrStride];
     const uint32_t nparts = cnt <= a.frag_size ? 1u : (cnt + a.frag_size - 1) / a.frag_size;
     if (b >= nparts_all && nparts > 1) return;  // rendered through its parts
-    const int j = search_last(a.job_tile_base, 0, a.njobs - 1, tile);
+    const int j = job_lookup(a.tile_hint, a.job_tile_base, a.njobs, tile);
     const JobDesc& J = a.jobs[j];
     const bool persp = J.persp != 0;
     const int local = tile - J.tile_base;
@@ -1078,7 +1093,8 @@ __global__ __launch_bounds__(kShadeBlock, 7) void k_shade(BatchArgs a) {
   __shared__ int64_t dcache[kMaxLdsDraws];
   const int tid = threadIdx.x;
   const int b = (int)xcd_remap(blockIdx.x, a.shade_blocks);
-  const int j = __builtin_amdgcn_readfirstlane(search_last(a.job_shade_base, 0, a.njobs - 1, b));
+  const int j = __builtin_amdgcn_readfirstlane(
+      job_lookup(a.shade_hint, a.job_shade_base, a.njobs, b));
   const JobDesc J = uniform_job(a, j);
   const int bxn = (J.W + kShadeW - 1) / kShadeW;
   const int local = b - J.shade_base;

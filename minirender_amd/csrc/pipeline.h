@@ -52,6 +52,10 @@ struct BatchArgs {
   const JobDesc* jobs;
   const int32_t* job_tile_base;   // [njobs]
   const int32_t* job_shade_base;  // [njobs]
+  // Job hints: entry i = {job of tile (resp. shading block) i * kHintSpan,
+  // first tile (block) of the next job or INT32_MAX}.
+  const int2* tile_hint;          // [ceil(ntiles / kHintSpan)]
+  const int2* shade_hint;         // [ceil(shade_blocks / kHintSpan)]
   int32_t njobs;
   int64_t total_tris;
   int32_t ntiles;
