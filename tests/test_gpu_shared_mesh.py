@@ -124,3 +124,23 @@ def test_shared_upload_errors():
         assert lib.mr_mesh_release(ctx.h, base) == -5
         assert lib.mr_mesh_release(ctx.h, h.value) == 0
         assert lib.mr_mesh_release(ctx.h, h2.value) == 0
+
+
+def test_served_pool_requests():
+    """Requests over a resident pool of split models (the service's
+    workload, small): served in batches, each result equals rendering the
+    request alone; the first matches the oracle."""
+    kinds = ("rgb8", "depth", "winner", "rgb")
+    pool = scenes.serving_pool(nmeshes=6, seed=3, tri_lo=2000, tri_hi=20000)
+    reqs = scenes.serving_requests(pool, 10, seed=4)
+    with mra.Context(0) as ctx:
+        alone = [ctx.render(f, kinds) for f in reqs]
+        with mra.Server(ctx, max_batch=4, slots=2) as srv:
+            tickets = [srv.submit(f, kinds) for f in reqs]
+            results = [srv.wait(t) for t in tickets]
+        for i, (out, info) in enumerate(results):
+            assert info.status == 0
+            for k in kinds:
+                assert np.array_equal(out[k].view(np.uint8), alone[i][k].view(np.uint8)), (i, k)
+        assert_parity(alone[0], oracle_py.render(reqs[0], kinds), "pool request 0")
+        assert (alone[0]["winner"] != mra.NO_WINNER).sum() > 1000
