@@ -71,8 +71,9 @@ struct BatchPlan {
   std::vector<int32_t> job_bases;  // tile bases of all jobs, then shade bases
   std::vector<int2> hints;          // job hints of the tiles, then of the shading blocks
   size_t tile_hints = 0;
-  // Vertex groups: draws of one job with the same mesh and modelview share
-  // their projected vertices.
+  // Vertex groups: draws of one job with the same attribute arrays (one
+  // mesh, or meshes sharing them) and modelview share their projected
+  // vertices.
   std::vector<int64_t> vg_begin;
   std::vector<int32_t> vg_draw;
   int64_t total_verts = 0;

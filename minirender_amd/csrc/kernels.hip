@@ -4,7 +4,8 @@
 //   k_clear   look-back words, tile counts and merge counters, when the
 //             scratch is new (afterwards the shading kernel leaves them zero)
 //   k_vertex  (a) eye-space transform and projection once per vertex of each
-//             (mesh, modelview) group of a job (Renderer.cpp:340-349, 191-203)
+//             (attribute arrays, modelview) group of a job (Renderer.cpp:340-349,
+//             191-203): the per-material meshes of a model share one group
 //   k_geom    (a)+(b) index gather, near clip, setup and cull of every input
 //             triangle (Renderer.cpp:351-380, 163-236).  Each wavefront
 //             compacts its survivors into its own region of the item stream
