@@ -123,6 +123,28 @@ __device__ inline int job_lookup(const int2* hint, const int32_t* base, int njob
   return j;
 }
 
+// Last index i in [0, N) with rs[i] <= q, for non-decreasing rs with
+// rs[0] <= q: a 4-ary search (three independent LDS reads per level), half
+// the dependent steps of a binary search.
+template <int N>
+__device__ inline int owner_search(const int* rs, int q) {
+  int lo = 0;
+  int n = N;
+#pragma unroll
+  for (int it = 0; it < 8; it++) {
+    if (n >= 4) {
+      const int s = n >> 2;
+      const int a1 = rs[lo + s], a2 = rs[lo + 2 * s], a3 = rs[lo + 3 * s];
+      lo += (a1 <= q ? s : 0) + (a2 <= q ? s : 0) + (a3 <= q ? s : 0);
+      n = s;
+    } else if (n == 2) {
+      if (rs[lo + 1] <= q) lo += 1;
+      n = 1;
+    }
+  }
+  return lo;
+}
+
 // Draw of flat triangle g in job J: the job's only draw (ncached < 0), the
 // job's draw table cached in LDS when it fits (ncached > 0), else searched in
 // global memory.
@@ -788,7 +810,7 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
       // edge (exact advance of the serial recurrence), then step across the
       // tile as the reference's column loop does (Renderer.cpp:238-269).
       for (int q = tid; q < total; q += NT) {
-        const int i = search_last(row_start, 0, NT - 1, q);
+        const int i = owner_search<NT>(row_start, q);
         const TriSetup& s = setups[i];
         const int row = max(s.by0, y0) + (q - row_start[i]);
         const int cx0 = max(s.bx0, x0), cx1 = min(s.bx1, x0 + TW - 1);
