@@ -687,9 +687,22 @@ __global__ __launch_bounds__(256, 8) void k_bin(BatchArgs a) {
       const bool act = j + lane < cnt;
       BinItem it = first;
       if (j > 0) it = act ? a.items[r * kRegion + j + lane] : BinItem{0, 0, 0, 0, 0, 0};
-      const DrawDesc& D = a.draws[act ? it.draw : 0u];
+      // The job's tile grid, through scalar loads when the wavefront's items
+      // share one draw (the common case).
+      // (Lane 0 is active in every step: j < cnt.)
+      const uint32_t d0 = __builtin_amdgcn_readfirstlane(it.draw);
+      const uint32_t dd = act ? it.draw : d0;
+      int tbase, tiles_x;
+      if (__ballot(dd != d0) == 0ull) {
+        const DrawDesc Du = uniform_draw(a, (int)d0);
+        tbase = Du.tile_base;
+        tiles_x = Du.tiles_x;
+      } else {
+        tbase = a.draws[dd].tile_base;
+        tiles_x = a.draws[dd].tiles_x;
+      }
       ckpt_alloc(a, act, it, sw);
-      wave_tile_pairs<true>(a, act, it.id, it.draw, D.tile_base, D.tiles_x, it.bx0 >> sw,
+      wave_tile_pairs<true>(a, act, it.id, it.draw, tbase, tiles_x, it.bx0 >> sw,
                             it.bx1 >> sw, it.by0 >> sh, it.by1 >> sh, reserve);
     }
     r = rn;
