@@ -506,7 +506,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
   __shared__ u64 sh_w[kScanBlock / 64];
   __shared__ u64 sh_excl;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (tid == 0) sh_blk = (int)atomicAdd(&a.counters[kScanTicket], 1ull);
+  // (A single workgroup needs no ticket: it is workgroup 0.)
+  if (tid == 0) sh_blk = a.scan_blocks == 1 ? 0 : (int)atomicAdd(&a.counters[kScanTicket], 1ull);
   __syncthreads();
   const int blk = sh_blk;
   const int base = blk * (kScanBlock * kScanItems) + tid * kScanItems;
@@ -579,7 +580,12 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
     ptot += sh_p[w];
     ltot += sh_l[w];
   }
-  if (tid == 0) {
+  if (tid == 0 && a.scan_blocks == 1) {  // the only workgroup: bases 0, totals stored
+    a.counters[kHeavy] = htot;
+    a.counters[kParts] = ptot;
+    a.counters[kListed] = ltot;
+    sh_hbase = sh_pbase = sh_lbase = 0;
+  } else if (tid == 0) {
     sh_hbase = htot ? atomicAdd(&a.counters[kHeavy], (u64)htot) : 0;
     sh_pbase = ptot ? atomicAdd(&a.counters[kParts], (u64)ptot) : 0;
     sh_lbase = ltot ? atomicAdd(&a.counters[kListed], (u64)ltot) : 0;
