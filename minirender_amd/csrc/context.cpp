@@ -289,7 +289,7 @@ static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins
   if ((rc = dev_alloc(c, s.tile_off, nt, true))) return rc;
   if ((rc = dev_alloc(c, s.tile_cursor, nt * kCtrStride, true))) return rc;
   if ((rc = dev_alloc(c, s.tile_heavy, nt, true))) return rc;
-  if ((rc = dev_alloc(c, s.tile_list, nt, true))) return rc;
+  if ((rc = dev_alloc(c, s.tile_list, nt * 4, true))) return rc;
   if ((rc = dev_alloc(c, s.tile_done, nt, true))) return rc;
   if ((rc = dev_alloc(c, s.parts, parts * 8, true))) return rc;
   if ((rc = dev_alloc(c, s.bins, (size_t)bins_cap * 8, true))) return rc;
@@ -319,7 +319,7 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.tile_off = (uint32_t*)s.tile_off.p;
   a.tile_cursor = (uint32_t*)s.tile_cursor.p;
   a.tile_heavy = (uint32_t*)s.tile_heavy.p;
-  a.tile_list = (uint32_t*)s.tile_list.p;
+  a.tile_list = (uint4*)s.tile_list.p;
   a.tile_done = (uint32_t*)s.tile_done.p;
   a.parts = (uint2*)s.parts.p;
   a.parts_cap = (uint32_t)std::min<size_t>(s.parts.bytes / 8, 0xFFFFFFF0u);

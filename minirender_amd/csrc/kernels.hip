@@ -613,7 +613,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
         hs++;
         ps += nparts;
       } else if (v[i] > 0) {
-        a.tile_list[ls++] = (uint32_t)t;
+        a.tile_list[ls++] = make_uint4((uint32_t)t, o, v[i], 0u);
       } else {
         slot = kEmptyTile;  // shading writes its pixels' background
       }
@@ -757,21 +757,27 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
       a.counters[kHeavyOverflow] ? 0 : (int64_t)min(a.counters[kParts], (u64)a.parts_cap);
   const int64_t b = blockIdx.x;
   int tile, part;
+  uint32_t cnt, off;  // the tile's bin entries: number and first
   if (b < nparts_all) {
     const uint2 pp = a.parts[b];
     tile = (int)pp.x;
     part = (int)pp.y;
+    cnt = a.tile_count[(size_t)tile * kCtrStride];
+    off = a.tile_off[tile];
   } else {
-    // Tiles rastered whole come from the scan's list; tiles without bin
-    // entries are not in it (shading writes their background).
+    // Tiles rastered whole come from the scan's list, with their bin range
+    // (one load fewer before the bin entries); tiles without bin entries
+    // are not in it (shading writes their background).
     const int64_t t = b - nparts_all;
     const int64_t nlisted = (int64_t)min(a.counters[kListed], (u64)a.ntiles);
     if (t >= nlisted) return;
-    tile = (int)a.tile_list[xcd_remap(t, nlisted)];
+    const uint4 le = a.tile_list[xcd_remap(t, nlisted)];
+    tile = (int)le.x;
     part = 0;
+    off = le.y;
+    cnt = le.z;
   }
   {
-    const uint32_t cnt = a.tile_count[(size_t)tile * kCtrStride];
     const uint32_t nparts = cnt <= a.frag_size ? 1u : (cnt + a.frag_size - 1) / a.frag_size;
     if (b >= nparts_all && nparts > 1) return;  // rendered through its parts
     const int j = job_lookup(a.tile_hint, a.job_tile_base, a.njobs, tile);
@@ -781,7 +787,6 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
     const int tyi = local / J.tiles_x, txi = local - tyi * J.tiles_x;
     const int x0 = txi * TW, y0 = tyi * TH;
     for (int p = tid; p < TH * KS; p += NT) keys[p] = ~0ull;
-    const uint32_t off = a.tile_off[tile];
     uint32_t avail = cnt;  // entries actually written (bin-list overflow)
     if ((u64)off >= a.bins_cap) avail = 0;
     else if ((u64)off + cnt > a.bins_cap) avail = (uint32_t)(a.bins_cap - off);

@@ -16,7 +16,8 @@ struct BatchArgs {
   uint32_t* tile_off;        // [ntiles] exclusive scan of tile_count
   uint32_t* tile_cursor;     // [ntiles] scatter cursors
   uint32_t* tile_heavy;      // [ntiles] merge slot of a split tile, kNoSlot, or kEmptyTile
-  uint32_t* tile_list;       // [ntiles] tiles rastered whole (first counters[kListed])
+  uint4* tile_list;          // [ntiles] {tile, first bin entry, entries, 0} of the tiles
+                             // rastered whole (first counters[kListed])
   uint32_t* tile_done;       // [ntiles] finished parts of a split tile
   uint2* parts;              // [parts_cap] (tile, part) of split tiles
   uint32_t parts_cap;

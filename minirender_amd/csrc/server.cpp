@@ -75,7 +75,7 @@ uint64_t request_bytes(mr_ctx* c, const mr_frame& f, const mr_outputs& o) {
   }
   const uint64_t tiles = ((uint64_t)(f.width + c->tile_w - 1) / c->tile_w) *
                          ((uint64_t)(f.height + c->tile_h - 1) / c->tile_h);
-  return b + tris * 64 + tiles * 148;  // + 16 B per triangle: checkpoint bases
+  return b + tris * 64 + tiles * 160;  // + 16 B per triangle: checkpoint bases
 }
 
 }  // namespace
