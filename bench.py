@@ -11,8 +11,10 @@ a-time path (mr_render) with per-stage HIP events on the library's stream;
 `value` is input triangles per second over the timed region.
 
 Beside it, in the same JSON line:
-  roofline      the dominant kernel's algorithmic HBM bytes per launch over
-                its average event-timed duration, against 8 TB/s;
+  roofline      the frame's algorithmic HBM bytes (inputs read, RGB8 + depth
+                written) over the GPU time of its kernels (HIP events on the
+                library's stream), against 8 TB/s, with the HBM traffic the
+                counters measured per frame and the dominant kernel;
   cpu_baseline  the oracle (single-threaded C restatement of the reference
                 renderer) on the same frame on this host, a bounded sample;
   batched       256 independent jobs (1k-500k triangles, 1-8 materials,
@@ -21,8 +23,14 @@ Beside it, in the same JSON line:
                 against the same jobs one mr_render at a time;
   serving       a Poisson stream over 64 resident meshes (10k-1M triangles)
                 through the continuous-batching service, p50/p99 latency
-                (arrival to outputs in host memory) and frames/s, next to the
-                same stream through a one-request-at-a-time service.
+                (arrival to outputs in HBM, and in pinned host memory) and
+                frames/s, next to the same stream through a one-request-at-a-
+                time service and through the batched service replaying
+                captured HIP graphs; the highest rate each sustains within
+                the p99 budget;
+  target        the same comparison for a stream of 1920x1080 frames of one
+                resident 1M-triangle grid (orbit camera and point light per
+                request).
 """
 from __future__ import annotations
 
@@ -59,6 +67,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-serving", action="store_true")
+    p.add_argument("--no-target", action="store_true",
+                   help="skip the 1M-triangle 1080p service comparison")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run 5 device copies of 256 MiB (to scale HBM byte counters)")
@@ -229,22 +239,37 @@ def plain_section(args, torch, mra, frame, wl_name):
     dom = max(range(1, 6), key=lambda i: stage[i])
     dname = STAGES[dom]
     dms = float(stage[dom])
-    achieved = alg[dname] / (dms * 1e-3) / 1e9 if dms > 0 else 0.0
     fb = frame_alg_bytes(frame)
     fmed = float(np.median(fms))
-    roof = {"bound": "hbm", "kernel": KERNEL_OF[dname], "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "alg_bytes_per_launch": alg[dname], "avg_launch_ms": round(dms, 5),
-            "traffic": read_traffic(KERNEL_OF[dname], wl_name),
-            "frame": {"alg_bytes": fb, "gpu_ms": round(fmed, 5),
-                      "achieved": round(fb / (fmed * 1e-3) / 1e9, 1),
-                      "frac": round(fb / (fmed * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    # Headline: the frame's algorithmic bytes (positions, normals, uvs,
+    # indices, texels in; RGB8 + depth out) over the GPU time of its six
+    # kernels (events around the launches on the library's stream).  The
+    # dominant kernel moves only intermediate data (0 algorithmic bytes),
+    # so it is reported beside the headline with its counter traffic.
+    ktraffic = {}
+    for n in STAGES[1:]:
+        parts = [read_traffic(k, wl_name) for k in KERNEL_OF[n].split("+")]
+        ktraffic[KERNEL_OF[n]] = sum(parts) if all(v is not None for v in parts) else None
+    frame_traffic = (sum(ktraffic.values()) if all(v is not None for v in ktraffic.values())
+                     else None)
+    achieved = fb / (fmed * 1e-3) / 1e9 if fmed > 0 else 0.0
+    roof = {"bound": "hbm", "kernel": "frame (k_vertex, k_geom, k_scan, k_bin, k_tile, k_shade)",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "alg_bytes_per_launch": fb, "avg_launch_ms": round(fmed, 5),
+            "traffic": frame_traffic,
+            "traffic_over_alg": round(frame_traffic / fb, 2) if frame_traffic else None}
+    dk = {"kernel": KERNEL_OF[dname], "avg_launch_ms": round(dms, 5),
+          "alg_bytes_per_launch": alg[dname], "traffic": ktraffic.get(KERNEL_OF[dname])}
     # How hard the dominant kernel drives HBM: its measured traffic (from the
     # committed counter summary) over its launch time.
-    if roof["traffic"] and dms > 0:
-        tgbs = roof["traffic"] / (dms * 1e-3) / 1e9
-        roof["traffic_GB/s"] = round(tgbs, 1)
-        roof["traffic_frac"] = round(tgbs / HBM_PEAK_GBS, 4)
+    if dk["traffic"] and dms > 0:
+        tgbs = dk["traffic"] / (dms * 1e-3) / 1e9
+        dk["traffic_GB/s"] = round(tgbs, 1)
+        dk["traffic_frac"] = round(tgbs / HBM_PEAK_GBS, 4)
+    roof["dominant"] = dk
+    for name in STAGES[1:]:
+        kernels[KERNEL_OF[name]]["traffic"] = ktraffic.get(KERNEL_OF[name])
     res = {"ms_per_step": dt * 1e3, "value": frame.ntri / dt / 1e6, "frames_per_s": 1.0 / dt,
            "tile": tile, "autotune": tune, "kernels": kernels, "roofline": roof,
            "records": st.records, "bin_entries": st.bin_entries,
@@ -266,7 +291,7 @@ def cpu_section(args, frame, wl):
             break
     per = el / n
     res = {"value": round(frame.ntri / per / 1e6, 3), "unit": "Mtris/s", "cores": 1,
-           "kind": "port", "frames_per_s": round(1.0 / per, 4),
+           "kind": "port", "frames_per_s": round(1.0 / per, 4), "host": host_cpu(),
            "sample": f"{n} frames of the bench workload ({wl}) after 1 warm-up frame, "
                      f"{el:.1f} s, one thread, oracle/liboracle.so (-O3 -ffp-contract=off)"}
     return res
@@ -286,19 +311,36 @@ def _cpu_worker(job):
     return n, time.perf_counter() - t0
 
 
+def host_cpu():
+    """The host's CPU model, online CPUs and the CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
 def cpu_multicore(args, frame, per_frame_s):
     """Throughput over the host's cores: one independent renderer per process
-    (started before this process touches the GPU)."""
+    (started before this process touches the GPU).  The process count is the
+    CPUs this job may use, at most 16 (one GPU's share of the host)."""
     import multiprocessing as mp
-    procs = max(1, min(16, len(os.sched_getaffinity(0))))
+    host = host_cpu()
+    procs = max(1, min(16, host["affinity"]))
     n = max(2, int(args.cpu_seconds / 2 / per_frame_s))
     with mp.get_context("spawn").Pool(procs) as pool:
         res = pool.map(_cpu_worker, [(args.workload, n)] * procs)
     rate = sum(k / el for k, el in res)
     return {"value": round(frame.ntri * rate / 1e6, 3), "unit": "Mtris/s", "cores": procs,
-            "frames_per_s": round(rate, 4),
+            "frames_per_s": round(rate, 4), "host": host,
             "sample": f"{procs} processes x {n} frames after 1 warm-up frame each, "
-                      f"{max(el for _, el in res):.1f} s"}
+                      f"{max(el for _, el in res):.1f} s, on {procs} of {host['nproc']} "
+                      f"CPUs ({host['model']})"}
 
 
 def batched_section(args, torch, mra):
@@ -409,14 +451,15 @@ def serve_batch_profile(args, torch, mra):
             "stage_ms": {k: round(float(v), 4) for k, v in zip(STAGES, med)}}
 
 
-def serving_section(args, torch, mra):
-    from minirender_amd import scenes
+def serve_study(args, torch, mra, reqs, graphs=True):
+    """Poisson streams of `reqs` (in order) through the render service:
+    p50/p99 latency and frames/s at --serve-rate with outputs in HBM and in
+    pinned host memory, then the highest offered rate each mode sustains with
+    p99 within --p99-budget (doubling from 500 requests/s, then five
+    bisections).  Modes: plain (one request per batch, one slot), batched
+    (--serve-batch per batch, --serve-slots slots) and, with `graphs`, the
+    batched service replaying captured HIP graphs."""
     from minirender_amd._capi import mr_frame, mr_outputs
-    t_gen = time.perf_counter()
-    pool = scenes.serving_pool()
-    reqs = scenes.serving_requests(pool, args.serve_requests)
-    log(f"serving: pool of {len(pool)} meshes, {len(reqs)} requests, "
-        f"generated in {time.perf_counter() - t_gen:.1f} s")
     ctx = mra.Context(0)
     dev = torch.device("cuda:0")
     # Framebuffers, one set per resolution shared by all requests of that
@@ -456,8 +499,11 @@ def serving_section(args, torch, mra):
     med = sorted(range(k), key=lambda i: reqs[i].ntri)[k // 2]
     tunes = {"plain": ctx.autotune([reqs[med]], outs=[o_dev[med]], reps=5),
              "batched": ctx.autotune(reqs[:k], outs=o_dev[:k], reps=2)}
+    tunes["batched_graphs"] = tunes["batched"]
     log(f"serving: {tunes}")
-    modes = {"plain": (1, 1), "batched": (args.serve_batch, args.serve_slots)}
+    modes = {"plain": (1, 1, False), "batched": (args.serve_batch, args.serve_slots, False)}
+    if graphs:
+        modes["batched_graphs"] = (args.serve_batch, args.serve_slots, True)
 
     def trace(mode, arr, count, where):
         cfg = tunes[mode]
@@ -465,19 +511,24 @@ def serving_section(args, torch, mra):
         ctx.set_raster_threads(cfg["raster_threads"])
         ctx.set_frag_size(cfg["frag_size"])
         ctx.set_bin_factor(cfg["bin_factor"])
-        max_batch, slots = modes[mode]
-        with mra.Server(ctx, max_batch=max_batch, slots=slots,
-                        hbm_budget=32 * (1 << 30)) as srv:
-            lat, mk = srv.run_trace(farr, list(arr[:count]), oarr[where])
+        max_batch, slots, use_graphs = modes[mode]
+        ctx.set_graphs(use_graphs)
+        try:
+            with mra.Server(ctx, max_batch=max_batch, slots=slots,
+                            hbm_budget=32 * (1 << 30)) as srv:
+                lat, mk = srv.run_trace(farr, list(arr[:count]), oarr[where])
+        finally:
+            ctx.set_graphs(False)
         return {"p50_ms": round(float(np.percentile(lat, 50)), 3),
                 "p99_ms": round(float(np.percentile(lat, 99)), 3),
                 "frames_per_s": round(count / mk, 1), "requests": count,
                 "offered_rate": round(float(count / max(arr[count - 1], 1e-9)), 1)}
 
     trace("batched", np.zeros(n), min(n, 1024), "device")  # warm
-    out = {"rate": args.serve_rate, "pool_meshes": len(pool), "hbm_budget_gb": 32,
-           "p99_budget_ms": args.p99_budget, "autotune": tunes,
-           "modes": {k: {"max_batch": v[0], "slots": v[1]} for k, v in modes.items()}}
+    out = {"rate": args.serve_rate, "hbm_budget_gb": 32, "p99_budget_ms": args.p99_budget,
+           "autotune": tunes,
+           "modes": {k: {"max_batch": v[0], "slots": v[1], "graphs": v[2]}
+                     for k, v in modes.items()}}
     for where in ("device", "host"):
         res = {}
         count = n if where == "device" else min(n, 10000)
@@ -485,14 +536,13 @@ def serving_section(args, torch, mra):
             res[name] = trace(name, arrivals(args.serve_rate, count), count, where)
             log(f"serving {where} {name}: {res[name]}")
         out[where] = res
-    # Highest offered rate each mode sustains with p99 within the budget
-    # (device framebuffers): doubling from 500 req/s, then five bisections.
+
     def sustains(mode, rate):
         count = int(min(n, max(2000, rate * 1.5)))
         r = trace(mode, arrivals(rate, count), count, "device")
         r["rate"] = round(rate, 1)
         ok = r["p99_ms"] <= args.p99_budget and r["frames_per_s"] >= 0.9 * rate
-        log(f"sweep {r}: {'ok' if ok else 'over'}")
+        log(f"sweep {mode} {r}: {'ok' if ok else 'over'}")
         return ok, r
 
     cap = {}
@@ -515,10 +565,37 @@ def serving_section(args, torch, mra):
                     hi = mid
         cap[name] = best
     out["at_p99_budget"] = cap
-    if cap.get("plain") and cap.get("batched"):
-        out["speedup_at_p99_budget"] = round(cap["batched"]["frames_per_s"] /
-                                             cap["plain"]["frames_per_s"], 3)
+    for name in modes:
+        if name != "plain" and cap.get("plain") and cap.get(name):
+            out[f"speedup_at_p99_budget_{name}"] = round(
+                cap[name]["frames_per_s"] / cap["plain"]["frames_per_s"], 3)
     ctx.close()
+    return out
+
+
+def serving_section(args, torch, mra):
+    """The mixed service: 64 resident meshes of 10k-1M triangles (1-8
+    materials), per-request camera, light and resolution."""
+    from minirender_amd import scenes
+    t_gen = time.perf_counter()
+    pool = scenes.serving_pool()
+    reqs = scenes.serving_requests(pool, args.serve_requests)
+    log(f"serving: pool of {len(pool)} meshes, {len(reqs)} requests, "
+        f"generated in {time.perf_counter() - t_gen:.1f} s")
+    out = serve_study(args, torch, mra, reqs)
+    out["pool_meshes"] = len(pool)
+    return out
+
+
+def target_section(args, torch, mra):
+    """The 1M-triangle grid at 1920x1080 (Blinn-Phong), one resident mesh, a
+    camera on a sphere of radius 2-5 and a point light per request."""
+    from minirender_amd import scenes
+    reqs = scenes.megatri_requests(args.serve_requests)
+    log(f"target: {len(reqs)} requests of {reqs[0].ntri} triangles at 1920x1080")
+    out = serve_study(args, torch, mra, reqs)
+    out["workload"] = "1M-triangle grid 708^2, 1920x1080, per-request orbit camera + point light"
+    out["triangles"] = reqs[0].ntri
     return out
 
 
@@ -579,6 +656,9 @@ def main():
         log(f"batched: {res['batched']}")
     if not args.no_serving:
         res["serving"] = serving_section(args, torch, mra)
+    if not args.no_target:
+        res["target"] = target_section(args, torch, mra)
+        log(f"target: {res['target']}")
     print(json.dumps(res), flush=True)
 
 

@@ -2,7 +2,8 @@
 // include/minirender/Renderer.h:9-64).  Same methods and meaning; render()
 // runs the HIP pipeline of minirender_amd through its C-ABI
 // (include/minirender_amd.h).  Meshes and textures stay resident on the GPU
-// between frames; the getters return host copies of the last frame.
+// between frames, and so does the last frame: each getter copies back only
+// the image it returns.
 #ifndef MINIRENDER_RENDERER_H
 #define MINIRENDER_RENDERER_H
 
@@ -84,12 +85,21 @@ class Renderer {
                   std::vector<mr_draw>& out);
   void fillFrame(mr_frame& f, const std::vector<mr_draw>& draws);
   void flushImmediate();
+  bool ensureDeviceOutputs();
+  void freeDeviceOutputs();
+  void fetch(int kind);
 
   Array2<Vec3> _image;
   Array2<float> _depth;
   Array2<Vec3> _pnormals;
   std::vector<uint8_t> _rgb8;
   std::vector<uint32_t> _winners;
+  // render() writes into device framebuffers; a getter copies back its own
+  // image when the host copy is stale (_fresh false).
+  enum { kRGB8, kDepth, kWinner, kRGB, kNormals, kOutputs };
+  void* _devOut[kOutputs] = {};
+  size_t _devPixels = 0;
+  bool _fresh[kOutputs] = {true, true, true, true, true};
   Matrix4 _view, _projection;
   Vec3 _light, _bgcolor;
   float _ambient = 0.1f;

@@ -33,7 +33,7 @@ struct TriMesh;
 
 // Defaults of Scene.cpp:65-71.  `opacity` is carried but not rendered, as
 // in the reference.
-struct Material {
+struct Material : public RefCounted {
   Vec3 diffuse{0.7f, 0.7f, 0.9f};
   Vec3 specular{0.8f, 0.8f, 0.8f};
   Vec3 emissive{0, 0, 0};
@@ -50,7 +50,7 @@ struct Renderable {
   Renderable(TriMesh* m, const Matrix4& t) : mesh(m), transform(t) {}
 };
 
-struct SceneNode {
+struct SceneNode : public RefCounted {
   bool visible = true;  // not read by the renderer (as in the reference)
   Matrix4 transform = Matrix4::identity();
   Array<Shared<SceneNode>> children;
@@ -81,7 +81,7 @@ struct TriMesh : public Shape {
 struct Scene : public SceneNode {
   float ambientLight = 0.1f;  // Scene.cpp:73-76
   Vec3 light;                 // not read by the renderer (as in the reference)
-  void add(const Shared<SceneNode>& node) { children.push_back(node); }
+  Scene& add(const Shared<SceneNode>& node) { children << node; return *this; }
 };
 
 }  // namespace minirender

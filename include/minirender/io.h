@@ -32,7 +32,7 @@ bool saveXYZ(const Array2<Vec3>& points, const std::string& filename,
 // it is not an X3D document with a Scene).
 Shared<SceneNode> loadX3D(const std::string& filename);
 // x3d.cpp:17-33: fan triangulation of -1-terminated polygons.
-std::vector<int> triangulateIndices(const std::vector<int>& indices);
+Array<int> triangulateIndices(const Array<int>& indices);
 
 }  // namespace minirender
 

@@ -4,23 +4,166 @@
 // minirender.  Matrix4 is row-major: m(i, j) is row i, column j; m * v is
 // the affine transform ((m00 x + m01 y) + m02 z) + m03 per row, as the
 // reference's Renderer.cpp:345 uses it.
+//
+// The containers keep the idioms the reference's callers are written in
+// (samples/bench.cpp:17-48, 98; samples/main.cpp:108-112):
+//   Shared<TriMesh> mesh = new TriMesh();   a raw pointer becomes shared
+//   mesh->vertices << p << q;               append
+//   mesh->normalsI = mesh->indices;         copies share storage (shallow)
+//   a.length(), a.dup() / a.clone()         size, deep copy
 #ifndef MINIRENDER_VECMATH_H
 #define MINIRENDER_VECMATH_H
 
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <initializer_list>
 #include <memory>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
 namespace minirender {
 
-template <class T>
-using Shared = std::shared_ptr<T>;
+// Objects that Shared<> may wrap more than once from a raw pointer (scene
+// nodes, materials): the second wrap joins the first's ownership instead of
+// starting a second one.
+struct RefCounted : public std::enable_shared_from_this<RefCounted> {
+  virtual ~RefCounted() {}
+};
 
+// Shared ownership of a heap object, as asl::Shared: constructible from a
+// raw pointer (which it then owns), from a Shared or std::shared_ptr of a
+// derived type, and from nullptr.  Two Shared made from the same raw
+// pointer of a RefCounted object share one count.
 template <class T>
-using Array = std::vector<T>;
+class Shared {
+ public:
+  Shared() = default;
+  Shared(std::nullptr_t) {}
+  template <class U, class = typename std::enable_if<std::is_convertible<U*, T*>::value>::type>
+  Shared(U* p) : p_(adopt(p)) {}
+  template <class U, class = typename std::enable_if<std::is_convertible<U*, T*>::value>::type>
+  Shared(const std::shared_ptr<U>& p) : p_(p) {}
+  template <class U, class = typename std::enable_if<std::is_convertible<U*, T*>::value>::type>
+  Shared(const Shared<U>& p) : p_(p.shared()) {}
+
+  T* operator->() const { return p_.get(); }
+  T& operator*() const { return *p_; }
+  T* get() const { return p_.get(); }
+  T* ptr() const { return p_.get(); }
+  explicit operator bool() const { return (bool)p_; }
+  operator std::shared_ptr<T>() const { return p_; }
+  const std::shared_ptr<T>& shared() const { return p_; }
+  template <class U>
+  bool operator==(const Shared<U>& b) const { return get() == b.get(); }
+  template <class U>
+  bool operator!=(const Shared<U>& b) const { return get() != b.get(); }
+  bool operator==(std::nullptr_t) const { return !p_; }
+  bool operator!=(std::nullptr_t) const { return (bool)p_; }
+
+ private:
+  template <class U>
+  static std::shared_ptr<T> adopt(U* p) {
+    if (!p) return nullptr;
+    if constexpr (std::is_base_of<RefCounted, U>::value) {
+      std::shared_ptr<RefCounted> owner = static_cast<RefCounted*>(p)->weak_from_this().lock();
+      if (owner) return std::shared_ptr<T>(owner, static_cast<T*>(p));
+    }
+    return std::shared_ptr<T>(p);
+  }
+  std::shared_ptr<T> p_;
+};
+
+template <class T, class... A>
+Shared<T> makeShared(A&&... a) {
+  return Shared<T>(std::make_shared<T>(std::forward<A>(a)...));
+}
+
+// Dynamic array with shared storage, as asl::Array: copies and assignments
+// share the elements (as the reference's per-material OBJ meshes share one
+// vertex array, io.cpp:313-315); dup() and clone() make independent copies.
+template <class T>
+class Array {
+ public:
+  using value_type = T;
+  using iterator = typename std::vector<T>::iterator;
+  using const_iterator = typename std::vector<T>::const_iterator;
+
+  Array() : d_(std::make_shared<std::vector<T>>()) {}
+  explicit Array(int n) : d_(std::make_shared<std::vector<T>>((size_t)std::max(n, 0))) {}
+  Array(std::initializer_list<T> v) : d_(std::make_shared<std::vector<T>>(v)) {}
+  Array(const std::vector<T>& v) : d_(std::make_shared<std::vector<T>>(v)) {}
+  Array(std::vector<T>&& v) : d_(std::make_shared<std::vector<T>>(std::move(v))) {}
+  template <class It>
+  Array(It first, It last) : d_(std::make_shared<std::vector<T>>(first, last)) {}
+
+  int length() const { return (int)d_->size(); }
+  size_t size() const { return d_->size(); }
+  bool empty() const { return d_->empty(); }
+  T* data() { return d_->data(); }
+  const T* data() const { return d_->data(); }
+  T& operator[](size_t i) { return (*d_)[i]; }
+  const T& operator[](size_t i) const { return (*d_)[i]; }
+  T& back() { return d_->back(); }
+  const T& back() const { return d_->back(); }
+  T& last() { return d_->back(); }
+  iterator begin() { return d_->begin(); }
+  iterator end() { return d_->end(); }
+  const_iterator begin() const { return d_->begin(); }
+  const_iterator end() const { return d_->end(); }
+
+  Array& operator<<(const T& x) { d_->push_back(x); return *this; }
+  void push_back(const T& x) { d_->push_back(x); }
+  template <class... A>
+  void emplace_back(A&&... a) { d_->emplace_back(std::forward<A>(a)...); }
+  Array& append(const Array& b) {
+    if (b.d_ == d_) {
+      const std::vector<T> copy = *b.d_;
+      d_->insert(d_->end(), copy.begin(), copy.end());
+    } else {
+      d_->insert(d_->end(), b.d_->begin(), b.d_->end());
+    }
+    return *this;
+  }
+  Array& resize(int n) { d_->resize((size_t)std::max(n, 0)); return *this; }
+  void reserve(size_t n) { d_->reserve(n); }
+  void clear() { d_->clear(); }
+  void removeAt(int i) { d_->erase(d_->begin() + i); }
+  template <class It>
+  void insert(iterator pos, It first, It last) { d_->insert(pos, first, last); }
+  void assign(size_t n, const T& v) { d_->assign(n, v); }
+  template <class It>
+  void assign(It first, It last) { d_->assign(first, last); }
+  int indexOf(const T& x) const {
+    for (size_t i = 0; i < d_->size(); i++)
+      if ((*d_)[i] == x) return (int)i;
+    return -1;
+  }
+
+  // dup(): this array takes an independent copy of its elements (later
+  // changes no longer reach arrays it shared them with); clone(): an
+  // independent copy.
+  Array& dup() {
+    d_ = std::make_shared<std::vector<T>>(*d_);
+    return *this;
+  }
+  Array clone() const { return Array(*d_); }
+  explicit operator bool() const { return !d_->empty(); }
+  // Element-wise comparison.
+  bool operator==(const Array& b) const { return d_ == b.d_ || *d_ == *b.d_; }
+  bool operator!=(const Array& b) const { return !(*this == b); }
+  bool operator==(const std::vector<T>& b) const { return *d_ == b; }
+  bool operator!=(const std::vector<T>& b) const { return *d_ != b; }
+  // True when both arrays view the same storage.
+  bool shares(const Array& b) const { return d_ == b.d_; }
+  const std::vector<T>& vec() const { return *d_; }
+  std::vector<T>& vec() { return *d_; }
+  operator const std::vector<T>&() const { return *d_; }
+
+ private:
+  std::shared_ptr<std::vector<T>> d_;
+};
 
 struct Vec2 {
   float x = 0, y = 0;
@@ -31,7 +174,11 @@ struct Vec2 {
   Vec2 operator*(float s) const { return Vec2(x * s, y * s); }
   float operator*(const Vec2& b) const { return x * b.x + y * b.y; }  // dot
   float operator^(const Vec2& b) const { return x * b.y - y * b.x; }  // cross
+  Vec2 operator/(float s) const { return Vec2(x / s, y / s); }
+  Vec2 operator-() const { return Vec2(-x, -y); }
   Vec2 perpend() const { return Vec2(-y, x); }
+  float length() const { return std::sqrt((*this) * (*this)); }
+  Vec2 normalized() const { return *this / length(); }
 };
 inline Vec2 operator*(float s, const Vec2& v) { return Vec2(s * v.x, s * v.y); }
 
@@ -49,6 +196,8 @@ struct Vec3 {
     return Vec3(y * b.z - z * b.y, z * b.x - x * b.z, x * b.y - y * b.x);
   }
   Vec3& operator+=(const Vec3& b) { x += b.x; y += b.y; z += b.z; return *this; }
+  Vec3& operator-=(const Vec3& b) { x -= b.x; y -= b.y; z -= b.z; return *this; }
+  Vec3& operator*=(float s) { x *= s; y *= s; z *= s; return *this; }
   Vec3& operator/=(float s) { x /= s; y /= s; z /= s; return *this; }
   float length() const { return std::sqrt((*this) * (*this)); }
   Vec3 normalized() const { return *this / length(); }

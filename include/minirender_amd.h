@@ -42,6 +42,10 @@ extern "C" {
 #define MR_TEX_RGB32F 1  /* 3 floats per texel, as Material::texture (Scene.h:42)         */
 
 /* ---- one draw = one Renderable (Scene.h:47-53) with its material ------- */
+/* mr_draw.flags */
+#define MR_DRAW_NO_NEAR_CLIP 1 /* no near-plane test or clip: the triangles of
+                                  Renderer::paintTriangle(a, b, c, world = false)
+                                  (Renderer.cpp:169) */
 typedef struct mr_draw {
   int32_t mesh;        /* handle from mr_mesh_upload                             */
   int32_t texture;     /* handle from mr_texture_upload, or -1 for none          */
@@ -51,6 +55,7 @@ typedef struct mr_draw {
   float specular[3];
   float emissive[3];
   float shininess;
+  int32_t flags;       /* MR_DRAW_* (0: drawn as paintMesh draws, with clipping) */
 } mr_draw;
 
 /* ---- one frame = one Renderer::render() call --------------------------- */

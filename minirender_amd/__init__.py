@@ -14,8 +14,9 @@ from typing import Dict, Iterable, List, Optional, Sequence
 import numpy as np
 
 from . import scenes
-from ._capi import (MR_TEX_RGB8, MR_TEX_RGB32F, NO_WINNER, MrError, check, load_library,
-                    mr_draw, mr_frame, mr_outputs, mr_stats)
+from ._capi import (MR_DRAW_NO_NEAR_CLIP, MR_ERR_TIMEOUT, MR_TEX_RGB8, MR_TEX_RGB32F,
+                    NO_WINNER, MrError, check, load_library, mr_draw, mr_frame, mr_outputs,
+                    mr_stats)
 from .scenes import Draw, FrameSpec, Material, Mesh, Texture
 
 __all__ = ["Context", "FrameSpec", "Draw", "Material", "Mesh", "Texture", "MrError",
@@ -66,6 +67,7 @@ def frame_struct(frame: FrameSpec, mesh_handle, tex_handle):
         s.specular = _f(d.material.specular, 3)
         s.emissive = _f(d.material.emissive, 3)
         s.shininess = float(np.float32(d.material.shininess))
+        s.flags = MR_DRAW_NO_NEAR_CLIP if d.no_clip else 0
     f = mr_frame()
     f.width, f.height = int(frame.width), int(frame.height)
     f.projection = _f(frame.projection, 16)
@@ -308,6 +310,11 @@ class Server:
         from ._capi import mr_request_info
         info = mr_request_info()
         rc = self.lib.mr_server_wait(self.h, ticket, timeout_ms, C.byref(info))
+        if rc == MR_ERR_TIMEOUT:
+            # Still live in the server: its worker will write into these
+            # output arrays later, so they stay referenced until a wait
+            # that completes.
+            check(rc, "mr_server_wait")
         out = self._pending.pop(ticket)[0]
         check(rc, "mr_server_wait")
         return out, info

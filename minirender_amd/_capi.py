@@ -17,6 +17,8 @@ ERRORS = {
     -5: "MR_ERR_HANDLE", -6: "MR_ERR_NOGPU", -7: "MR_ERR_INDEX", -8: "MR_ERR_DEVICE",
     -9: "MR_ERR_TIMEOUT", -10: "MR_ERR_CLOSED",
 }
+MR_ERR_TIMEOUT = -9
+MR_DRAW_NO_NEAR_CLIP = 1
 MR_TEX_RGB8 = 0
 MR_TEX_RGB32F = 1
 NO_WINNER = 0xFFFFFFFF
@@ -33,7 +35,7 @@ class mr_draw(C.Structure):
         ("mesh", C.c_int32), ("texture", C.c_int32),
         ("modelview", C.c_float * 16), ("normalmat", C.c_float * 16),
         ("diffuse", C.c_float * 3), ("specular", C.c_float * 3),
-        ("emissive", C.c_float * 3), ("shininess", C.c_float),
+        ("emissive", C.c_float * 3), ("shininess", C.c_float), ("flags", C.c_int32),
     ]
 
 

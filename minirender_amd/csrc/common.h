@@ -40,6 +40,8 @@ struct DrawDesc {
   uint32_t order_base;  // 2 * triangles of earlier draws of the same job
   int32_t tiles_x;
   int64_t vbase;        // first projected-vertex record of this draw's mesh
+  int32_t no_clip;      // 1: no near-plane test (paintTriangle, world = false)
+  int32_t pad_;
 };
 
 // One frame of a batch.

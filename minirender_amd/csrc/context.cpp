@@ -195,6 +195,7 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
         D.emissive[i] = s.emissive[i];
       }
       D.shininess = s.shininess;
+      D.no_clip = (s.flags & MR_DRAW_NO_NEAR_CLIP) ? 1 : 0;
       D.pos = (const float*)va.pos.p;
       D.nrm = (const float*)va.nrm.p;
       D.idx = (const int32_t*)m.idx.p;

@@ -1,5 +1,8 @@
 // cpp_api.cpp -- the minirender C++ API (include/minirender/*.h) over the
-// C-ABI of this library.  Host code only.
+// C-ABI of this library.  Host code only (the HIP runtime API is used for the
+// Renderer's device framebuffers).
+#include <hip/hip_runtime_api.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -333,7 +336,50 @@ Renderer::Renderer() {  // Renderer.cpp:85-98
 }
 
 Renderer::~Renderer() {
+  freeDeviceOutputs();
   if (_ctx) mr_ctx_destroy(_ctx);
+}
+
+// Device framebuffers of render(): the frame stays on the GPU and each
+// getter copies back only its own image.
+void Renderer::freeDeviceOutputs() {
+  for (void*& p : _devOut)
+    if (p) {
+      hipFree(p);
+      p = nullptr;
+    }
+  _devPixels = 0;
+}
+
+bool Renderer::ensureDeviceOutputs() {
+  const size_t n = (size_t)_image.cols() * _image.rows();
+  if (_devPixels == n && _devOut[0]) return true;
+  freeDeviceOutputs();
+  const size_t bytes[kOutputs] = {n * 3, n * 4, n * 4, n * 12, n * 12};
+  for (int k = 0; k < kOutputs; k++)
+    if (hipMalloc(&_devOut[k], bytes[k] ? bytes[k] : 16) != hipSuccess) {
+      (void)hipGetLastError();
+      freeDeviceOutputs();
+      _status = MR_ERR_NOMEM;
+      return false;
+    }
+  _devPixels = n;
+  return true;
+}
+
+// Copy device output k to the host image if the host copy is stale.
+void Renderer::fetch(int k) {
+  if (_fresh[k] || !_devOut[k] || _devPixels == 0) return;
+  const size_t n = _devPixels;
+  void* dst[kOutputs] = {_rgb8.data(), _depth.data(), _winners.data(), _image.data(),
+                         _pnormals.data()};
+  const size_t bytes[kOutputs] = {n * 3, n * 4, n * 4, n * 12, n * 12};
+  if (hipMemcpy(dst[k], _devOut[k], bytes[k], hipMemcpyDeviceToHost) != hipSuccess) {
+    (void)hipGetLastError();
+    _status = MR_ERR_HIP;
+    return;
+  }
+  _fresh[k] = true;
 }
 
 mr_ctx* Renderer::context() {
@@ -351,6 +397,7 @@ void Renderer::setSize(int w, int h) {  // Renderer.cpp:100-106
 }
 
 void Renderer::clear() {  // Renderer.cpp:113-119
+  for (bool& f : _fresh) f = true;  // the host images below are the frame
   _immediate.clear();
   _immediateMeshes.clear();
   _image.set(_bgcolor);
@@ -371,7 +418,7 @@ void Renderer::clear() {  // Renderer.cpp:113-119
 // FNV-1a over sizes, addresses and (for large arrays, sampled) contents: a
 // changed or reallocated mesh is uploaded again.
 template <class T>
-static void hashArray(uint64_t& h, const std::vector<T>& v) {
+static void hashArray(uint64_t& h, const Array<T>& v) {
   auto mix = [&](const void* p, size_t n) {
     const unsigned char* c = (const unsigned char*)p;
     for (size_t i = 0; i < n; i++) h = (h ^ c[i]) * 1099511628211ull;
@@ -390,7 +437,8 @@ static void hashArray(uint64_t& h, const std::vector<T>& v) {
 }
 
 template <class T>
-static bool sameBytes(const std::vector<T>& a, const std::vector<T>& b) {
+static bool sameBytes(const Array<T>& a, const Array<T>& b) {
+  if (a.shares(b)) return true;
   return a.size() == b.size() && (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(T)) == 0);
 }
 
@@ -555,14 +603,17 @@ void Renderer::render() {
   _immediateMeshes.clear();
   const mr_frame& f = describeFrame();
   mr_ctx* c = context();
-  if (!c) return;
+  if (!c || !ensureDeviceOutputs()) return;
   mr_outputs o{};
-  o.rgb = (float*)_image.data();
-  o.depth = _depth.data();
-  o.rgb8 = _rgb8.data();
-  o.winner = _winners.data();
-  o.normals = _saveNormals ? (float*)_pnormals.data() : nullptr;
+  o.rgb8 = (uint8_t*)_devOut[kRGB8];
+  o.depth = (float*)_devOut[kDepth];
+  o.winner = (uint32_t*)_devOut[kWinner];
+  o.rgb = (float*)_devOut[kRGB];
+  o.normals = _saveNormals ? (float*)_devOut[kNormals] : nullptr;
+  o.on_device = 1;
   _status = mr_render(c, &f, &o);
+  for (int k = 0; k < kOutputs; k++) _fresh[k] = false;
+  if (!_saveNormals) _fresh[kNormals] = true;  // not produced: keeps its last contents
 }
 
 void Renderer::paintMesh(TriMesh* mesh, const Matrix4& transform) {
@@ -570,9 +621,10 @@ void Renderer::paintMesh(TriMesh* mesh, const Matrix4& transform) {
   appendDraw(mesh, transform, *_material, _immediate);
 }
 
-void Renderer::paintTriangle(const Vertex& a, const Vertex& b, const Vertex& c, bool) {
+void Renderer::paintTriangle(const Vertex& a, const Vertex& b, const Vertex& c, bool world) {
   // Eye-space vertices: drawn with identity modelview through the same
-  // pipeline (clipped against the near plane in both modes).
+  // pipeline; world = false skips the near-plane test and clip
+  // (Renderer.cpp:169).
   auto m = std::make_shared<TriMesh>();
   for (const Vertex* v : {&a, &b, &c}) {
     m->vertices.push_back(v->position);
@@ -585,7 +637,9 @@ void Renderer::paintTriangle(const Vertex& a, const Vertex& b, const Vertex& c, 
   _immediateMeshes.push_back(m);
   const Matrix4 saved = _view;
   _view = Matrix4::identity();
+  const size_t before = _immediate.size();
   appendDraw(m.get(), Matrix4::identity(), *_material, _immediate);
+  if (!world && _immediate.size() > before) _immediate.back().flags = MR_DRAW_NO_NEAR_CLIP;
   _view = saved;
 }
 
@@ -593,6 +647,7 @@ void Renderer::flushImmediate() {
   if (_immediate.empty()) return;
   mr_ctx* c = context();
   if (!c) return;
+  for (int k = 0; k < kOutputs; k++) fetch(k);  // composited on the host
   const int W = _image.cols(), H = _image.rows();
   const size_t n = (size_t)W * H;
   std::vector<Vec3> rgb(n), nrm(_saveNormals ? n : 0);
@@ -630,14 +685,15 @@ void Renderer::flushImmediate() {
   _immediateMeshes.clear();
 }
 
-Array2<float> Renderer::getDepth() { flushImmediate(); return _depth; }
-Array2<Vec3> Renderer::getImage() { flushImmediate(); return _image; }
-Array2<Vec3> Renderer::getNormalsImage() { flushImmediate(); return _pnormals; }
-std::vector<uint8_t> Renderer::getImageRGB8() { flushImmediate(); return _rgb8; }
-std::vector<uint32_t> Renderer::getWinners() { flushImmediate(); return _winners; }
+Array2<float> Renderer::getDepth() { flushImmediate(); fetch(kDepth); return _depth; }
+Array2<Vec3> Renderer::getImage() { flushImmediate(); fetch(kRGB); return _image; }
+Array2<Vec3> Renderer::getNormalsImage() { flushImmediate(); fetch(kNormals); return _pnormals; }
+std::vector<uint8_t> Renderer::getImageRGB8() { flushImmediate(); fetch(kRGB8); return _rgb8; }
+std::vector<uint32_t> Renderer::getWinners() { flushImmediate(); fetch(kWinner); return _winners; }
 
 Array2<Vec3> Renderer::getRangeImage() {  // Renderer.cpp:388-415
   flushImmediate();
+  fetch(kDepth);
   const Matrix4& P = _projection;
   const bool persp = P(3, 3) == 0;
   float zn = 0;

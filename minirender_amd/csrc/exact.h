@@ -343,6 +343,17 @@ MR_HD inline ScreenVtx project_vertex(const float* P, bool persp, float w, float
   return v;
 }
 
+// Last pixel index the reference's loop `for (v = floor(lo) + 0.5f;
+// v <= hi + 0.5f; v++)` visits (Renderer.cpp:238,243), for 0 <= hi < 2^23.
+// Pixel centres k + 0.5f are exact, so the last one is the largest k + 0.5
+// not above fl(hi + 0.5f).  That sum rounds up when hi lies within half an
+// ulp of the result below a power of two (1023.99994f + 0.5f == 1024.5f):
+// the loop then visits floor(hi) + 1.  fl(hi + 0.5f) - 0.5f is exact here.
+MR_HD inline int last_center(float hi) {
+  const float end = hi + 0.5f;
+  return (int)floorf(end - 0.5f);
+}
+
 // Triangle setup of paintTriangle (Renderer.cpp:193-236) from projected
 // vertices.  Returns false when the triangle is rejected off-screen or
 // culled (front faces are counter-clockwise in NDC, :207).
@@ -371,9 +382,9 @@ MR_HD inline bool setup_screen(float w, float h, const ScreenVtx& v0, const Scre
   miny = aclamp(miny, 0.0f, h - 1);
   maxy = aclamp(maxy, 0.0f, h - 1);
   r.bx0 = (int)floorf(minx);
-  r.bx1 = (int)floorf(maxx);
+  r.bx1 = last_center(maxx);
   r.by0 = (int)floorf(miny);
-  r.by1 = (int)floorf(maxy);
+  r.by1 = last_center(maxy);
   r.z0 = v0.zv;
   r.z1 = v1.zv;
   r.z2 = v2.zv;

@@ -78,8 +78,8 @@ Shared<TriMesh> load_stl_ascii(const std::string& filename) {
   while (f >> tag) {
     if (tag == "endfacet") {
       if (np == 3) {
-        obj->indices.insert(obj->indices.end(), {iv - 3, iv - 2, iv - 1});
-        obj->normalsI.insert(obj->normalsI.end(), {in - 1, in - 1, in - 1});
+        obj->indices << iv - 3 << iv - 2 << iv - 1;
+        obj->normalsI << in - 1 << in - 1 << in - 1;
       }
       np = 0;
     } else if (tag == "normal" || tag == "vertex") {
@@ -122,8 +122,8 @@ Shared<TriMesh> load_stl_binary(const std::string& filename) {
     obj->normals.push_back(Vec3(v[0], v[1], v[2]));
     for (int j = 0; j < 3; j++) obj->vertices.push_back(Vec3(v[3 + 3 * j], v[4 + 3 * j], v[5 + 3 * j]));
     const int b = 3 * i;
-    obj->indices.insert(obj->indices.end(), {b, b + 1, b + 2});
-    obj->normalsI.insert(obj->normalsI.end(), {i, i, i});
+    obj->indices << b << b + 1 << b + 2;
+    obj->normalsI << i << i << i;
   }
   std::fclose(f);
   return obj;
@@ -132,15 +132,15 @@ Shared<TriMesh> load_stl_binary(const std::string& filename) {
 }  // namespace
 
 // Fan triangulation of -1-terminated polygons (reference x3d.cpp:17-33).
-std::vector<int> triangulateIndices(const std::vector<int>& idx) {
-  std::vector<int> tris;
+Array<int> triangulateIndices(const Array<int>& idx) {
+  Array<int> tris;
   tris.reserve(idx.size());
   int i = 0, j = 0;
   const int n = (int)idx.size();
   while (i < n) {
     for (j = i + 1; j < n - 1; j++) {
       if (idx[j] == -1 || idx[j + 1] == -1) break;
-      tris.insert(tris.end(), {idx[i], idx[j], idx[j + 1]});
+      tris << idx[i] << idx[j] << idx[j + 1];
     }
     i = j + 2;
   }
@@ -197,8 +197,10 @@ Shared<SceneNode> loadOBJ(const std::string& filename) {
   Shared<TriMesh> mesh = std::make_shared<TriMesh>();
   mesh->material = materials[""];
   meshes[""] = mesh;
-  std::vector<Vec3> vertices, normals;
-  std::vector<Vec2> texcoords;
+  // One set of attribute arrays, shared by the per-material meshes
+  // (io.cpp:313-315: assignment shares the storage).
+  Array<Vec3> vertices, normals;
+  Array<Vec2> texcoords;
   std::string line;
   while (std::getline(file, line)) {
     if (!line.empty() && line[0] == '#') continue;
@@ -273,14 +275,15 @@ Shared<SceneNode> loadOBJ(const std::string& filename) {
     m.vertices = vertices;
     m.normals = normals;
     m.texcoords = texcoords;
-    if (m.normals.empty()) {  // one normal per face
+    if (m.normals.empty()) {  // one normal per face, in arrays of this mesh
+      m.normals.dup();
       m.normalsI.clear();
       for (size_t i = 0, j = 0; i + 2 < m.indices.size(); i += 3, j++) {
         const Vec3 a = m.vertices[m.indices[i]];
         const Vec3 b = m.vertices[m.indices[i + 1]];
         const Vec3 c = m.vertices[m.indices[i + 2]];
         m.normals.push_back(((b - a) ^ (c - a)).normalized());
-        m.normalsI.insert(m.normalsI.end(), {(int)j, (int)j, (int)j});
+        m.normalsI << (int)j << (int)j << (int)j;
       }
     }
     node->children.push_back(e.second);

@@ -227,7 +227,7 @@ __device__ inline int tri_setup(const BatchArgs& a, const DrawDesc& D, const Job
   const float w = (float)J.W, h = (float)J.H;
   const bool b0 = e[0].z > zn, b1 = e[1].z > zn, b2 = e[2].z > zn;
   clipped = false;
-  if (b0 || b1 || b2) {
+  if (!D.no_clip && (b0 || b1 || b2)) {
     if (b0 && b1 && b2) return 0;
     clipped = true;
     f3 a0, a1, a2, c0, c1, c2;
@@ -255,10 +255,10 @@ __device__ inline int tri_setup(const BatchArgs& a, const DrawDesc& D, const Job
 
 // Setup of sub-triangle `sub` of a binned triangle from its corners.
 __device__ inline void tri_setup_corners(const JobDesc& J, const f3 e[3], const ScreenVtx v[3],
-                                         int sub, TriSetup& s) {
+                                         int sub, bool no_clip, TriSetup& s) {
   const float zn = J.znear;
   const float w = (float)J.W, h = (float)J.H;
-  if (e[0].z > zn || e[1].z > zn || e[2].z > zn) {
+  if (!no_clip && (e[0].z > zn || e[1].z > zn || e[2].z > zn)) {
     const bool persp = J.persp != 0;
     f3 a0, a1, a2, c0, c1, c2;
     clip_positions(zn, e[0], e[1], e[2], a0, a1, a2, c0, c1, c2);
@@ -275,7 +275,7 @@ __device__ inline void tri_setup_sub(const BatchArgs& a, const DrawDesc& D, cons
   f3 e[3];
   ScreenVtx v[3];
   corners(a, D, t, e, v);
-  tri_setup_corners(J, e, v, sub, s);
+  tri_setup_corners(J, e, v, sub, D.no_clip != 0, s);
 }
 
 // (a): kVertsPerLane vertices per lane of every vertex group: eye-space
@@ -974,7 +974,7 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   v1.p = ec[1];
   v2.p = ec[2];
   TriSetup s;
-  tri_setup_corners(J, ec, vc, sub, s);
+  tri_setup_corners(J, ec, vc, sub, D.no_clip != 0, s);
   // Edge values at the pixel, from the same serial walk along the row as the
   // raster's: from the raster's checkpoint where the row enters the pixel's
   // tile column, or from the bbox's left column.
@@ -1005,7 +1005,7 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   v0.uv = U0;
   v1.uv = U1;
   v2.uv = U2;
-  if (v0.p.z > J.znear || v1.p.z > J.znear || v2.p.z > J.znear) {
+  if (!D.no_clip && (v0.p.z > J.znear || v1.p.z > J.znear || v2.p.z > J.znear)) {
     Vtx a0, a1, a2, b0, b1, b2;
     clip_triangle(J.znear, v0, v1, v2, a0, a1, a2, b0, b1, b2);
     if (sub) { v0 = b0; v1 = b1; v2 = b2; }

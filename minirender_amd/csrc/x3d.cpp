@@ -351,14 +351,14 @@ class X3dReader {
       mesh->texcoords.push_back(Vec2(uvs[i], 1 - uvs[i + 1]));
     if (ifs) {
       mesh->indices = triangulateIndices(ints_of(g->get("coordIndex")));
-      const std::vector<int> ti = ints_of(g->get("texCoordIndex"));
-      const std::vector<int> ni = ints_of(g->get("normalIndex"));
-      mesh->texcoordsI = ti.empty() ? mesh->indices : triangulateIndices(ti);
-      mesh->normalsI = ni.empty() ? mesh->indices : triangulateIndices(ni);
+      const Array<int> ti = ints_of(g->get("texCoordIndex"));
+      const Array<int> ni = ints_of(g->get("normalIndex"));
+      mesh->texcoordsI = ti.empty() ? mesh->indices.clone() : triangulateIndices(ti);
+      mesh->normalsI = ni.empty() ? mesh->indices.clone() : triangulateIndices(ni);
     } else {
       mesh->indices = ints_of(g->get("index"));
-      mesh->texcoordsI = mesh->indices;
-      mesh->normalsI = mesh->indices;
+      mesh->texcoordsI = mesh->indices.clone();
+      mesh->normalsI = mesh->indices.clone();
     }
     const int nv = (int)mesh->vertices.size();
     for (int v : mesh->indices)
@@ -370,12 +370,12 @@ class X3dReader {
         const Vec3 b = mesh->vertices[mesh->indices[i + 1]];
         const Vec3 c = mesh->vertices[mesh->indices[i + 2]];
         mesh->normals.push_back(((b - a) ^ (c - a)).normalized());
-        mesh->normalsI.insert(mesh->normalsI.end(), {(int)j, (int)j, (int)j});
+        mesh->normalsI << (int)j << (int)j << (int)j;
       }
     }
     if (mesh->texcoords.empty()) {
       mesh->texcoords.push_back(Vec2(0, 0));
-      mesh->texcoordsI.assign(mesh->indices.size(), 0);
+      mesh->texcoordsI = Array<int>(mesh->indices.length());  // zeros
     }
     return mesh;
   }

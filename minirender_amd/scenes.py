@@ -97,6 +97,9 @@ class Draw:
     mesh: Mesh
     material: Material = field(default_factory=Material)
     transform: np.ndarray = field(default_factory=lambda: np.eye(4, dtype=F32))
+    # no near-plane test or clip, as Renderer::paintTriangle(a, b, c, false)
+    # draws (Renderer.cpp:169)
+    no_clip: bool = False
 
 
 @dataclass(eq=False)
@@ -528,6 +531,20 @@ def orbit_camera(rng: np.random.Generator, width: int, height: int, rmin=2.0, rm
     eye = (r * math.cos(ph) * math.cos(th), r * math.sin(ph), r * math.cos(ph) * math.sin(th))
     return dict(projection=projection_frustum(math.radians(45.0), width / height, 0.1, 100.0),
                 view=look_at(eye, (0.0, 0.0, 0.0)))
+
+
+def megatri_requests(n: int, seed: int = 13, width: int = 1920, height: int = 1080) -> List[FrameSpec]:
+    """`n` render requests of the 1M-triangle grid (one shared mesh), each with
+    its own camera on a sphere of radius 2-5 looking at the origin and its own
+    point light."""
+    base = config_megatri(width, height)
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        light = tuple(float(v) for v in rng.uniform(-4.0, 4.0, 3))
+        out.append(FrameSpec(width, height, draws=base.draws, light=light, light_is_point=True,
+                             ambient=0.1, **orbit_camera(rng, width, height)))
+    return out
 
 
 def split_materials(mesh: Mesh, parts: int, rng: np.random.Generator) -> List[Draw]:

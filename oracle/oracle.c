@@ -430,7 +430,10 @@ int mro_render(const mr_frame* f, const mro_mesh* meshes, int32_t nmeshes,
         }
       }
       if (rc != MR_OK) break;
-      paint(&s, &v[0], &v[1], &v[2], 1, base + 2u * (uint32_t)t);
+      /* paintMesh draws with world = true (:373-379); a draw flagged
+         MR_DRAW_NO_NEAR_CLIP holds paintTriangle(a, b, c, false) triangles */
+      paint(&s, &v[0], &v[1], &v[2], (dr->flags & MR_DRAW_NO_NEAR_CLIP) ? 0 : 1,
+            base + 2u * (uint32_t)t);
     }
     free(ev);
     free(en);

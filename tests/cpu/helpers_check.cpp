@@ -4,6 +4,11 @@
 //          and emit the same sub-triangles, bit for bit, as the reference's
 //          rotate-while-loop clipTriangle (Renderer.cpp:131-161), NaNs and
 //          ties included.
+//   bound  last_center (exact.h) is the last pixel centre the reference's
+//          fp32 loop `for (x = floor(lo) + 0.5f; x <= hi + 0.5f; x++)`
+//          visits (Renderer.cpp:238,243), for every float within 64 ulps of
+//          each power of two from 2 to 16384 and for random floats below
+//          16384 (where hi + 0.5f rounds up, the loop visits one more).
 // Built with g++ -O2 -ffp-contract=off.  Prints "ok" or the first failure.
 #include <cmath>
 #include <cstdint>
@@ -62,8 +67,48 @@ static int clip_loop(float z, Vtx v[3], Vtx out[6]) {
   return 2;
 }
 
+// The reference's loop, run from a few columns below hi: index of the last
+// centre visited.
+static int loop_last(float hi) {
+  float start = floorf(hi) - 3.0f;
+  if (start < 0.0f) start = 0.0f;
+  int last = -1;
+  for (float x = start + 0.5f; x <= hi + 0.5f; x++) last = (int)x;
+  return last;
+}
+
+static bool check_bound(float hi, long* rounded_up) {
+  const int want = loop_last(hi), got = last_center(hi);
+  if (want != got) {
+    printf("bound mismatch: hi=%a (%.9g) loop %d last_center %d\n", hi, hi, want, got);
+    return false;
+  }
+  if (want != (int)floorf(hi)) ++*rounded_up;
+  return true;
+}
+
 int main(int argc, char** argv) {
   const long cases = argc > 1 ? atol(argv[1]) : 200000;
+  // bound
+  long rounded_up = 0;
+  for (int k = 1; k <= 14; k++) {
+    const float p = ldexpf(1.0f, k);
+    float lo = p, hi = p;
+    for (int u = 0; u < 64; u++) {
+      lo = nextafterf(lo, 0.0f);
+      hi = nextafterf(hi, 1e30f);
+    }
+    for (float v = lo; v <= hi; v = nextafterf(v, 1e30f))
+      if (!check_bound(v, &rounded_up)) return 1;
+  }
+  for (long c = 0; c < cases; c++) {
+    const float v = (float)((double)(rnd() >> 11) / 9007199254740992.0 * 16384.0);
+    if (!check_bound(v, &rounded_up)) return 1;
+  }
+  if (rounded_up == 0) {
+    printf("bound: no case where the loop visits floor(hi) + 1\n");
+    return 1;
+  }
   // remap
   for (int64_t n = 1; n <= 4096; n++) {
     std::vector<char> seen(n, 0);

@@ -108,3 +108,21 @@ def stack_frame(layers: int = 1500, width: int = 80, height: int = 56, seed: int
     mat = Material(diffuse=(0.6, 0.5, 0.4), specular=(0.3, 0.3, 0.3), shininess=8.0)
     return FrameSpec(width, height, proj, draws=[Draw(m, mat)], light=(1.0, 2.0, 3.0),
                      light_is_point=True)
+
+
+def bound_frame() -> FrameSpec:
+    """One triangle whose rightmost vertex projects to x = 1024 - 2^-14 (the
+    largest float below 1024) and whose other vertices lie 2^25 pixels to
+    the left.  The reference's column loop runs while x <= pmax.x + 0.5f in
+    fp32 (Renderer.cpp:238,243), and 1023.99994f + 0.5f rounds to 1024.5f,
+    so it visits column 1024.  Along row 8 the edge step n1.x is below half
+    an ulp of e1, so e1 stays below 1 and e2 stays 0: pixel (8, 1024) is
+    covered.  The orthographic projection maps world x, y to pixel x,
+    16 - y exactly."""
+    F = np.float32
+    x1 = F(1024) - F(2.0 ** -14)
+    pos = np.array([[-2.0 ** 25, 7.5, 0.0], [x1, 7.5, 0.0], [-2.0 ** 25, 14.0, 0.0]], F)
+    nrm = np.tile(np.array([[0.0, 0.0, 1.0]], F), (3, 1))
+    m = scenes.Mesh(pos, nrm, np.array([[0, 1, 2]], np.int32))
+    return FrameSpec(2048, 16, scenes.projection_ortho(0, 2048, 0, 16, -1, 1),
+                     draws=[Draw(m, Material(diffuse=(0.9, 0.3, 0.1)))], lighting=False)

@@ -7,7 +7,10 @@ reference's, built with g++ from the same headers the HIP kernels use.
                  random, tie-heavy and special-value cases.
   helpers_check  xcd_remap (common.h) is a bijection; the branch-free near
                  clip equals the reference's rotate-loop clipTriangle
-                 (Renderer.cpp:131-161) bit for bit, NaNs and ties included.
+                 (Renderer.cpp:131-161) bit for bit, NaNs and ties included;
+                 the bbox end bound last_center equals the last pixel of the
+                 reference's fp32 loop x <= pmax + 0.5f (Renderer.cpp:238,243)
+                 near every power of two from 2 to 16384.
 """
 import subprocess
 from pathlib import Path

@@ -261,3 +261,18 @@ def test_autotune_and_cache(tmp_path):
             assert t2[k] == t1[k]
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_bbox_end_bound_rounds_up(gpu_ctx, tile):
+    """The last column is the last pixel centre of the reference's fp32 loop
+    x <= pmax.x + 0.5f (Renderer.cpp:243): with pmax.x = 1024 - 2^-14 that
+    is column 1024, which this scene covers (binning, raster and shading
+    all use the same bound)."""
+    from tests.parity import bound_frame
+    gpu_ctx.set_tile(tile, tile)
+    try:
+        g, o = check(gpu_ctx, bound_frame(), f"bbox end bound t{tile}")
+    finally:
+        gpu_ctx.set_tile(16, 16)
+    assert g["winner"][8, 1024] == 0

@@ -291,3 +291,17 @@ def test_sphere_top_fan_is_back_facing():
     o = oracle_py.render(fr, ("winner",))
     w = o["winner"][o["winner"] != 0xFFFFFFFF] // 2
     assert w.size > 1000 and (w >= 64).all()
+
+
+def test_column_bound_rounds_like_the_reference_loop():
+    """pmax.x = 1024 - 2^-14: the fp32 loop bound pmax.x + 0.5f rounds to
+    1024.5f (Renderer.cpp:243), so column 1024 is visited and, in this
+    scene, covered (e1 stuck below 1, e2 == 0 along row 8).  A bound of
+    floor(pmax.x) would end at column 1023."""
+    from tests.parity import bound_frame
+    o = oracle_py.render(bound_frame(), ("winner", "depth"))
+    w = o["winner"]
+    assert w[8, 1024] == 0
+    assert w[8, 1025] == 0xFFFFFFFF
+    assert o["depth"][8, 1024] == np.float32(0.0)
+    assert np.float32(np.float32(1024) - np.float32(2.0 ** -14)) + np.float32(0.5) == np.float32(1024.5)
