@@ -79,6 +79,8 @@ def parse():
     p.add_argument("--p99-budget", type=float, default=50.0, help="ms")
     p.add_argument("--serve-batch-profile", action="store_true",
                    help="only time one batch of --serve-batch serving requests (GPU stages)")
+    p.add_argument("--target-batch-profile", action="store_true",
+                   help="only time one batch of --serve-batch 1M-triangle 1080p requests")
     return p.parse_args()
 
 
@@ -398,15 +400,19 @@ def batched_section(args, torch, mra):
             "speedup": tp / tb}
 
 
-def serve_batch_profile(args, torch, mra):
+def serve_batch_profile(args, torch, mra, target=False):
     """One batch of the service's requests (--serve-batch of them over the
-    resident pool, device outputs), rendered --steps times with the batch's
-    autotuned launch configuration: wall time per batch and the GPU time of
-    each stage (HIP events, in a separate profiled pass)."""
+    resident pool, or of the 1M-triangle stream with `target`, device
+    outputs), rendered --steps times with the batch's autotuned launch
+    configuration: wall time per batch and the GPU time of each stage (HIP
+    events, in a separate profiled pass)."""
     from minirender_amd import scenes
     from minirender_amd._capi import mr_frame, mr_outputs
-    pool = scenes.serving_pool()
-    reqs = scenes.serving_requests(pool, args.serve_batch)
+    if target:
+        reqs = scenes.megatri_requests(args.serve_batch)
+    else:
+        pool = scenes.serving_pool()
+        reqs = scenes.serving_requests(pool, args.serve_batch)
     ctx = mra.Context(0)
     dev = torch.device("cuda:0")
     structs, keep, outs, bufs = [], [], [], []
@@ -616,10 +622,11 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n, "copies": 5}), flush=True)
         return
-    if args.serve_batch_profile:
+    if args.serve_batch_profile or args.target_batch_profile:
         import torch
         import minirender_amd as mra
-        print(json.dumps(serve_batch_profile(args, torch, mra)), flush=True)
+        print(json.dumps(serve_batch_profile(args, torch, mra, args.target_batch_profile)),
+              flush=True)
         return
     frame, wl = make_frame(args.workload)
     log(f"workload {args.workload}: {frame.ntri} triangles")

@@ -72,7 +72,8 @@ class Renderer {
  private:
   struct MeshEntry {
     int32_t handle = -1;
-    uint64_t fingerprint = 0;
+    uint64_t fingerprint = 0;       // content hash at the last upload
+    uint64_t stamps[12] = {};       // (storage id, version) of the six arrays then
     const TriMesh* mesh = nullptr;
   };
   struct TexEntry {

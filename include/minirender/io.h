@@ -14,6 +14,15 @@
 namespace minirender {
 
 bool savePPM(const Array2<Vec3>& image, const std::string& filename);
+
+// The console output of the reference's render sample (samples/main.cpp:
+// 13-28): cursor to the top left, then one space per pixel on a background
+// of the pixel's colour, int(colour * 255) per channel, a line per row.
+// consoleImage returns that text (24-bit colour escapes, or the nearest of
+// the 6x6x6 cube of 256-colour terminals with colors256); consolePaint
+// writes it to stdout.
+std::string consoleImage(const Array2<Vec3>& image, bool colors256 = false);
+void consolePaint(const Array2<Vec3>& image, bool colors256 = false);
 Array2<Vec3> loadPPM(const std::string& filename);
 
 // io.cpp:114-133: .stl -> a node holding the mesh, .obj -> loadOBJ,

@@ -15,7 +15,9 @@
 #define MINIRENDER_VECMATH_H
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdint>
 #include <cstddef>
 #include <initializer_list>
 #include <memory>
@@ -80,64 +82,77 @@ Shared<T> makeShared(A&&... a) {
   return Shared<T>(std::make_shared<T>(std::forward<A>(a)...));
 }
 
+// Identity of array storage: every storage gets a new id when it is created.
+inline std::atomic<uint64_t> g_array_storage_ids{1};
+
 // Dynamic array with shared storage, as asl::Array: copies and assignments
 // share the elements (as the reference's per-material OBJ meshes share one
 // vertex array, io.cpp:313-315); dup() and clone() make independent copies.
+// The storage carries an id and a modification version: every non-const
+// access (which may write) bumps the version, so a consumer that kept
+// (id(), version()) knows the contents are unchanged when both still match.
+// Writes through a pointer or reference obtained before that point are not
+// seen.
 template <class T>
 class Array {
+  struct Store {
+    std::vector<T> v;
+    uint64_t id = g_array_storage_ids.fetch_add(1, std::memory_order_relaxed);
+    uint64_t version = 0;
+    Store() = default;
+    explicit Store(std::vector<T> x) : v(std::move(x)) {}
+  };
+
  public:
   using value_type = T;
   using iterator = typename std::vector<T>::iterator;
   using const_iterator = typename std::vector<T>::const_iterator;
 
-  Array() : d_(std::make_shared<std::vector<T>>()) {}
-  explicit Array(int n) : d_(std::make_shared<std::vector<T>>((size_t)std::max(n, 0))) {}
-  Array(std::initializer_list<T> v) : d_(std::make_shared<std::vector<T>>(v)) {}
-  Array(const std::vector<T>& v) : d_(std::make_shared<std::vector<T>>(v)) {}
-  Array(std::vector<T>&& v) : d_(std::make_shared<std::vector<T>>(std::move(v))) {}
+  Array() : d_(std::make_shared<Store>()) {}
+  explicit Array(int n) : d_(std::make_shared<Store>(std::vector<T>((size_t)std::max(n, 0)))) {}
+  Array(std::initializer_list<T> v) : d_(std::make_shared<Store>(std::vector<T>(v))) {}
+  Array(const std::vector<T>& v) : d_(std::make_shared<Store>(v)) {}
+  Array(std::vector<T>&& v) : d_(std::make_shared<Store>(std::move(v))) {}
   template <class It>
-  Array(It first, It last) : d_(std::make_shared<std::vector<T>>(first, last)) {}
+  Array(It first, It last) : d_(std::make_shared<Store>(std::vector<T>(first, last))) {}
 
-  int length() const { return (int)d_->size(); }
-  size_t size() const { return d_->size(); }
-  bool empty() const { return d_->empty(); }
-  T* data() { return d_->data(); }
-  const T* data() const { return d_->data(); }
-  T& operator[](size_t i) { return (*d_)[i]; }
-  const T& operator[](size_t i) const { return (*d_)[i]; }
-  T& back() { return d_->back(); }
-  const T& back() const { return d_->back(); }
-  T& last() { return d_->back(); }
-  iterator begin() { return d_->begin(); }
-  iterator end() { return d_->end(); }
-  const_iterator begin() const { return d_->begin(); }
-  const_iterator end() const { return d_->end(); }
+  int length() const { return (int)d_->v.size(); }
+  size_t size() const { return d_->v.size(); }
+  bool empty() const { return d_->v.empty(); }
+  const T* data() const { return d_->v.data(); }
+  const T& operator[](size_t i) const { return d_->v[i]; }
+  const T& back() const { return d_->v.back(); }
+  const_iterator begin() const { return d_->v.begin(); }
+  const_iterator end() const { return d_->v.end(); }
+  // Non-const access: the caller may write.
+  T* data() { return w().data(); }
+  T& operator[](size_t i) { return w()[i]; }
+  T& back() { return w().back(); }
+  T& last() { return w().back(); }
+  iterator begin() { return w().begin(); }
+  iterator end() { return w().end(); }
 
-  Array& operator<<(const T& x) { d_->push_back(x); return *this; }
-  void push_back(const T& x) { d_->push_back(x); }
+  Array& operator<<(const T& x) { w().push_back(x); return *this; }
+  void push_back(const T& x) { w().push_back(x); }
   template <class... A>
-  void emplace_back(A&&... a) { d_->emplace_back(std::forward<A>(a)...); }
+  void emplace_back(A&&... a) { w().emplace_back(std::forward<A>(a)...); }
   Array& append(const Array& b) {
-    if (b.d_ == d_) {
-      const std::vector<T> copy = *b.d_;
-      d_->insert(d_->end(), copy.begin(), copy.end());
-    } else {
-      d_->insert(d_->end(), b.d_->begin(), b.d_->end());
-    }
+    const std::vector<T> copy = b.d_->v;
+    w().insert(d_->v.end(), copy.begin(), copy.end());
     return *this;
   }
-  Array& resize(int n) { d_->resize((size_t)std::max(n, 0)); return *this; }
-  void reserve(size_t n) { d_->reserve(n); }
-  void clear() { d_->clear(); }
-  void removeAt(int i) { d_->erase(d_->begin() + i); }
+  Array& resize(int n) { w().resize((size_t)std::max(n, 0)); return *this; }
+  void reserve(size_t n) { d_->v.reserve(n); }
+  void clear() { w().clear(); }
+  void removeAt(int i) { w().erase(d_->v.begin() + i); }
   template <class It>
-  void insert(iterator pos, It first, It last) { d_->insert(pos, first, last); }
-  void assign(size_t n, const T& v) { d_->assign(n, v); }
+  void insert(iterator pos, It first, It last) { w().insert(pos, first, last); }
+  void assign(size_t n, const T& v) { w().assign(n, v); }
   template <class It>
-  void assign(It first, It last) { d_->assign(first, last); }
+  void assign(It first, It last) { w().assign(first, last); }
   int indexOf(const T& x) const {
-    for (size_t i = 0; i < d_->size(); i++)
-      if ((*d_)[i] == x) return (int)i;
+    for (size_t i = 0; i < d_->v.size(); i++)
+      if (d_->v[i] == x) return (int)i;
     return -1;
   }
 
@@ -145,24 +160,31 @@ class Array {
   // changes no longer reach arrays it shared them with); clone(): an
   // independent copy.
   Array& dup() {
-    d_ = std::make_shared<std::vector<T>>(*d_);
+    d_ = std::make_shared<Store>(d_->v);
     return *this;
   }
-  Array clone() const { return Array(*d_); }
-  explicit operator bool() const { return !d_->empty(); }
+  Array clone() const { return Array(d_->v); }
+  explicit operator bool() const { return !d_->v.empty(); }
   // Element-wise comparison.
-  bool operator==(const Array& b) const { return d_ == b.d_ || *d_ == *b.d_; }
+  bool operator==(const Array& b) const { return d_ == b.d_ || d_->v == b.d_->v; }
   bool operator!=(const Array& b) const { return !(*this == b); }
-  bool operator==(const std::vector<T>& b) const { return *d_ == b; }
-  bool operator!=(const std::vector<T>& b) const { return *d_ != b; }
+  bool operator==(const std::vector<T>& b) const { return d_->v == b; }
+  bool operator!=(const std::vector<T>& b) const { return d_->v != b; }
+
   // True when both arrays view the same storage.
   bool shares(const Array& b) const { return d_ == b.d_; }
-  const std::vector<T>& vec() const { return *d_; }
-  std::vector<T>& vec() { return *d_; }
-  operator const std::vector<T>&() const { return *d_; }
+  uint64_t id() const { return d_->id; }
+  uint64_t version() const { return d_->version; }
+  const std::vector<T>& vec() const { return d_->v; }
+  std::vector<T>& vec() { return w(); }
+  operator const std::vector<T>&() const { return d_->v; }
 
  private:
-  std::shared_ptr<std::vector<T>> d_;
+  std::vector<T>& w() {
+    ++d_->version;
+    return d_->v;
+  }
+  std::shared_ptr<Store> d_;
 };
 
 struct Vec2 {

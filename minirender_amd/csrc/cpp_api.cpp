@@ -415,24 +415,26 @@ void Renderer::clear() {  // Renderer.cpp:113-119
   }
 }
 
-// FNV-1a over sizes, addresses and (for large arrays, sampled) contents: a
-// changed or reallocated mesh is uploaded again.
+// Hash of an array's size and every byte, eight bytes per step.
 template <class T>
 static void hashArray(uint64_t& h, const Array<T>& v) {
-  auto mix = [&](const void* p, size_t n) {
-    const unsigned char* c = (const unsigned char*)p;
-    for (size_t i = 0; i < n; i++) h = (h ^ c[i]) * 1099511628211ull;
+  const size_t bytes = v.size() * sizeof(T);
+  const unsigned char* p = (const unsigned char*)v.data();
+  auto mix = [&](uint64_t w) {
+    h = (h ^ w) * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 29;
   };
-  const size_t n = v.size();
-  const void* ptr = v.data();
-  mix(&n, sizeof(n));
-  mix(&ptr, sizeof(ptr));
-  if (n <= 65536) {
-    mix(v.data(), n * sizeof(T));
-  } else {
-    const size_t step = n / 4096;
-    for (size_t i = 0; i < n; i += step) mix(&v[i], sizeof(T));
-    mix(&v[n - 1], sizeof(T));
+  mix(bytes);
+  size_t i = 0;
+  for (; i + 8 <= bytes; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, p + i, 8);
+    mix(w);
+  }
+  if (i < bytes) {
+    uint64_t w = 0;
+    std::memcpy(&w, p + i, bytes - i);
+    mix(w);
   }
 }
 
@@ -446,7 +448,26 @@ static bool usesTexcoords(const TriMesh* m) {
   return !m->texcoords.empty() && m->texcoordsI.size() == m->indices.size();
 }
 
-int32_t Renderer::meshHandle(TriMesh* m) {
+// Storage ids and modification versions of a mesh's six arrays.
+static void meshStamps(const TriMesh* m, uint64_t out[12]) {
+  const uint64_t v[12] = {m->vertices.id(),  m->vertices.version(), m->normals.id(),
+                          m->normals.version(), m->texcoords.id(), m->texcoords.version(),
+                          m->indices.id(),   m->indices.version(), m->normalsI.id(),
+                          m->normalsI.version(), m->texcoordsI.id(), m->texcoordsI.version()};
+  for (int i = 0; i < 12; i++) out[i] = v[i];
+}
+
+int32_t Renderer::meshHandle(TriMesh* mesh) {
+  // All reads below go through a const view: they leave the arrays'
+  // modification versions alone.
+  const TriMesh* m = mesh;
+  MeshEntry& e = _meshes[m];
+  uint64_t stamps[12];
+  meshStamps(m, stamps);
+  if (e.handle >= 0 && std::memcmp(stamps, e.stamps, sizeof(stamps)) == 0) {
+    if (_sharing) _frameMeshes.push_back(m);  // untouched since its upload
+    return e.handle;
+  }
   uint64_t fp = 1469598103934665603ull;
   hashArray(fp, m->vertices);
   hashArray(fp, m->normals);
@@ -454,7 +475,7 @@ int32_t Renderer::meshHandle(TriMesh* m) {
   hashArray(fp, m->indices);
   hashArray(fp, m->normalsI);
   hashArray(fp, m->texcoordsI);
-  MeshEntry& e = _meshes[m];
+  std::memcpy(e.stamps, stamps, sizeof(stamps));
   if (e.handle >= 0 && e.fingerprint == fp) {
     if (_sharing) _frameMeshes.push_back(m);
     return e.handle;

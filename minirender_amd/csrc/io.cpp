@@ -320,3 +320,40 @@ bool saveXYZ(const Array2<Vec3>& points, const std::string& filename, const Matr
 }
 
 }  // namespace minirender
+
+namespace minirender {
+
+// samples/main.cpp:13-28: home the cursor, then per pixel a background-colour
+// escape and a space.  The channel value is int(v * 255) (truncation, no
+// clamping, as Vec3::with<int>()); escapes take it clamped to 0..255.
+std::string consoleImage(const Array2<Vec3>& image, bool colors256) {
+  auto chan = [](float v) {
+    const int c = (int)(v * 255);
+    return c < 0 ? 0 : (c > 255 ? 255 : c);
+  };
+  std::string out = "\x1b[H";
+  out.reserve((size_t)image.rows() * image.cols() * 20 + 16);
+  char buf[48];
+  for (int i = 0; i < image.rows(); i++) {
+    for (int j = 0; j < image.cols(); j++) {
+      const Vec3& p = image(i, j);
+      const int r = chan(p.x), g = chan(p.y), b = chan(p.z);
+      if (colors256)
+        std::snprintf(buf, sizeof(buf), "\x1b[48;5;%dm ",
+                      16 + 36 * (r * 6 / 256) + 6 * (g * 6 / 256) + b * 6 / 256);
+      else
+        std::snprintf(buf, sizeof(buf), "\x1b[48;2;%d;%d;%dm ", r, g, b);
+      out += buf;
+    }
+    out += "\x1b[0m\n";
+  }
+  return out;
+}
+
+void consolePaint(const Array2<Vec3>& image, bool colors256) {
+  const std::string s = consoleImage(image, colors256);
+  std::fwrite(s.data(), 1, s.size(), stdout);
+  std::fflush(stdout);
+}
+
+}  // namespace minirender

@@ -7,8 +7,8 @@
 // generator here instead of asl::Random).
 //
 //   ref_idiom cpu            container semantics the idiom relies on (shared
-//                            storage, dup / clone, one object wrapped twice)
-//                            and the scene it builds
+//                            storage, dup / clone, one object wrapped twice),
+//                            the scene it builds and the console painter
 //   ref_idiom gpu            frames of such a scene through Renderer::render()
 //                            against the CPU oracle, and paintTriangle with
 //                            world = false (no near clip, Renderer.cpp:169)
@@ -30,6 +30,7 @@
 
 #include "minirender/Renderer.h"
 #include "minirender/Scene.h"
+#include "minirender/io.h"
 #include "oracle_compare.h"
 
 using namespace minirender;
@@ -158,6 +159,15 @@ static int cpu_checks() {
   auto size = box.size();
   auto center = box.center();
   CHECK(size.x > 0 && size.y > 0 && size.z > 0 && std::fabs(center.x) < 400, "bbox");
+
+  // Console output of the render sample (samples/main.cpp:13-28).
+  Array2<Vec3> img(1, 2);
+  img(0, 0) = Vec3(1, 0.5f, 0);
+  img(0, 1) = Vec3(2, -1, 0.1f);
+  CHECK(consoleImage(img) == "\x1b[H\x1b[48;2;255;127;0m \x1b[48;2;255;0;25m \x1b[0m\n",
+        "console image");
+  CHECK(consoleImage(img, true) == "\x1b[H\x1b[48;5;208m \x1b[48;5;196m \x1b[0m\n",
+        "console image, 256 colours");
   return 0;
 }
 

@@ -276,3 +276,29 @@ def test_bbox_end_bound_rounds_up(gpu_ctx, tile):
     finally:
         gpu_ctx.set_tile(16, 16)
     assert g["winner"][8, 1024] == 0
+
+
+def test_no_near_clip_draws(gpu_ctx):
+    """Draws flagged MR_DRAW_NO_NEAR_CLIP (Renderer::paintTriangle with
+    world = false, Renderer.cpp:169) skip the near test: a triangle with a
+    vertex between the eye and the near plane is drawn whole, and a sphere
+    around the camera is drawn with vertices behind the eye projected as
+    they are (no clipping, no near rejection)."""
+    F = np.float32
+    pos = np.array([[-1.0, -0.8, -2.0], [1.5, -0.6, -3.0], [0.2, 1.2, -0.3]], F)
+    tri = scenes.Mesh(pos, np.tile(np.array([[0, 0, 1]], F), (3, 1)), np.array([[0, 1, 2]], np.int32))
+    mat = Material(emissive=(0.9, 0.5, 0.1))
+    proj = scenes.projection_frustum(1.0, 96 / 64, 0.5, 50.0)
+    clipped = FrameSpec(96, 64, proj, draws=[Draw(tri, mat)], lighting=False)
+    whole = FrameSpec(96, 64, proj, draws=[Draw(tri, mat, no_clip=True)], lighting=False)
+    gc, _ = check(gpu_ctx, clipped, "near-clipped triangle")
+    gw, _ = check(gpu_ctx, whole, "unclipped triangle")
+    assert (gw["winner"] != gc["winner"]).sum() > 0
+    sph = scenes.create_sphere(2.0, 10, 20)
+    fr = FrameSpec(160, 120, scenes.projection_frustum(1.2, 4 / 3, 0.1, 20.0),
+                   view=scenes.translate(0.3, -0.2, 0.5),
+                   draws=[Draw(sph, Material(shininess=20.0), no_clip=True),
+                          Draw(scenes.create_cube(0.7), Material(diffuse=(0.2, 0.8, 0.3)),
+                               scenes.translate(0.0, 0.0, -1.5))],
+                   light=(0.5, 1.0, 2.0), light_is_point=True)
+    check(gpu_ctx, fr, "unclipped sphere around the eye")
