@@ -36,6 +36,7 @@ extern "C" {
 #define MR_ERR_DEVICE -8   /* a kernel reported an internal error (spin limit) */
 #define MR_ERR_TIMEOUT -9  /* a wait timed out                                   */
 #define MR_ERR_CLOSED -10  /* server is shutting down                            */
+#define MR_ERR_LIMIT -11   /* frame exceeds the fixed scratch of mr_render_noalloc */
 
 /* ---- texture formats ----------------------------------------------------- */
 #define MR_TEX_RGB8 0    /* 3 bytes per texel, texel value = byte / 255.0f (io.cpp:405-411) */
@@ -202,6 +203,19 @@ int mr_texture_release(mr_ctx* ctx, int32_t handle);
 /* Render one frame: Renderer::render() (Renderer.cpp:311-332) for the draws
  * of `frame`.  Synchronous: returns when outputs are written. */
 int mr_render(mr_ctx* ctx, const mr_frame* frame, const mr_outputs* out);
+
+/* mr_render that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Outputs
+ * must be device buffers (out->on_device = 1, else MR_ERR_ARG); meshes and
+ * textures must already be resident.  All scratch of the frame comes from one
+ * static device array of 64 MiB that each device the library runs on holds
+ * until the process ends (a 1280x720 frame of ~300 k triangles fits; larger
+ * frames get MR_ERR_LIMIT, a bin-list overflow MR_ERR_NOMEM instead of a
+ * re-run with a larger list).  Every call on a device shares that array:
+ * calls that overlap on one GPU (several threads or contexts) race.
+ * Synchronous, on the context's stream; results equal mr_render's. */
+int mr_render_noalloc(mr_ctx* ctx, const mr_frame* frame, const mr_outputs* out);
 
 /* Render n independent frames in one batched pass (all stages launched once
  * for the whole batch).  Equivalent to n calls of mr_render. */

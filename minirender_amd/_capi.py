@@ -15,7 +15,7 @@ MR_OK = 0
 ERRORS = {
     -1: "MR_ERR_ARG", -2: "MR_ERR_HIP", -3: "MR_ERR_NOMEM", -4: "MR_ERR_BUDGET",
     -5: "MR_ERR_HANDLE", -6: "MR_ERR_NOGPU", -7: "MR_ERR_INDEX", -8: "MR_ERR_DEVICE",
-    -9: "MR_ERR_TIMEOUT", -10: "MR_ERR_CLOSED",
+    -9: "MR_ERR_TIMEOUT", -10: "MR_ERR_CLOSED", -11: "MR_ERR_LIMIT",
 }
 MR_ERR_TIMEOUT = -9
 MR_DRAW_NO_NEAR_CLIP = 1
@@ -116,6 +116,7 @@ SIGNATURES = {
     "mr_texture_upload": (C.c_int, [_P, _P, _I, _I, _I, _IP]),
     "mr_texture_release": (C.c_int, [_P, _I]),
     "mr_render": (C.c_int, [_P, C.POINTER(mr_frame), C.POINTER(mr_outputs)]),
+    "mr_render_noalloc": (C.c_int, [_P, C.POINTER(mr_frame), C.POINTER(mr_outputs)]),
     "mr_render_batch": (C.c_int, [_P, C.POINTER(mr_frame), _I, C.POINTER(mr_outputs)]),
     "mr_ctx_autotune": (C.c_int, [_P, C.POINTER(mr_frame), _I, C.POINTER(mr_outputs), _I,
                                   C.c_char_p, C.POINTER(mr_tuning)]),

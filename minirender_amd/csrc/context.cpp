@@ -274,36 +274,62 @@ int build_batch(mr_ctx* c, const mr_frame* frames, int nframes, BatchPlan& plan)
 // room; a batch that needs more leaves the extra rows to shading's own walk.
 static size_t ckpt_entries(int64_t npixels) { return (size_t)npixels * 2 + 65536; }
 
-static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins_cap,
-                          uint32_t heavy_slots) {
-  Scratch& s = sl.sc;
+// Device scratch of a batch, buffer by buffer (bytes), in the order of
+// scratch_bufs().
+constexpr int kScratchBufs = 20;
+
+static void scratch_sizes(mr_ctx* c, const BatchPlan& p, uint64_t bins_cap, uint32_t heavy_slots,
+                          size_t out[kScratchBufs]) {
   const int64_t T = p.total_tris;
   const int scb = scan_blocks_for((int32_t)p.ntiles);
   const size_t nt = (size_t)p.ntiles * 4;
   const int64_t nregions = std::max<int64_t>((T + 63) / 64, 1);
   const size_t npt = (size_t)c->tile_w * c->tile_h * 8;
   const size_t parts = (size_t)(bins_cap / c->frag_size) + heavy_slots + 16;
+  const size_t v[kScratchBufs] = {
+      (size_t)nregions * kRegion * sizeof(BinItem),  // items
+      (size_t)nregions * 4,                          // region_count
+      nt * kCtrStride,                               // tile_count
+      nt,                                            // tile_off
+      nt * kCtrStride,                               // tile_cursor
+      nt,                                            // tile_heavy
+      nt * 4,                                        // tile_list
+      nt,                                            // tile_done
+      parts * 8,                                     // parts
+      (size_t)bins_cap * 8,                          // bins
+      (size_t)std::max<int64_t>(2 * T, 1) * 8,       // ckpt_base
+      ckpt_entries(p.npixels) * 8,                   // ckpt
+      (size_t)kCkptSegs * kCtrStride * 8,            // ckpt_ctr
+      (size_t)heavy_slots * npt,                     // heavy_keys
+      (size_t)std::max(scb, 1) * 8,                  // scan_status
+      (size_t)kNumCounters * 8,                      // counters
+      (size_t)2 * kStatSlots * 8,                    // block_stats
+      p.layout.bytes,                                // tables
+      (size_t)p.total_verts * kVrecFloats * 4,       // vrec
+      (size_t)p.npixels * 4,                         // win
+  };
+  for (int k = 0; k < kScratchBufs; k++) out[k] = v[k];
+}
+
+static void scratch_bufs(Scratch& x, DevBuf* b[kScratchBufs]) {
+  DevBuf* v[kScratchBufs] = {&x.items, &x.region_count, &x.tile_count, &x.tile_off,
+                             &x.tile_cursor, &x.tile_heavy, &x.tile_list, &x.tile_done,
+                             &x.parts, &x.bins, &x.ckpt_base, &x.ckpt, &x.ckpt_ctr,
+                             &x.heavy_keys, &x.scan_status, &x.counters, &x.block_stats,
+                             &x.tables, &x.vrec, &x.win};
+  for (int k = 0; k < kScratchBufs; k++) b[k] = v[k];
+}
+
+static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins_cap,
+                          uint32_t heavy_slots) {
+  Scratch& s = sl.sc;
+  size_t bytes[kScratchBufs];
+  DevBuf* bufs[kScratchBufs];
+  scratch_sizes(c, p, bins_cap, heavy_slots, bytes);
+  scratch_bufs(s, bufs);
   int rc;
-  if ((rc = dev_alloc(c, s.items, (size_t)nregions * kRegion * sizeof(BinItem), true))) return rc;
-  if ((rc = dev_alloc(c, s.region_count, (size_t)nregions * 4, true))) return rc;
-  if ((rc = dev_alloc(c, s.tile_count, nt * kCtrStride, true))) return rc;
-  if ((rc = dev_alloc(c, s.tile_off, nt, true))) return rc;
-  if ((rc = dev_alloc(c, s.tile_cursor, nt * kCtrStride, true))) return rc;
-  if ((rc = dev_alloc(c, s.tile_heavy, nt, true))) return rc;
-  if ((rc = dev_alloc(c, s.tile_list, nt * 4, true))) return rc;
-  if ((rc = dev_alloc(c, s.tile_done, nt, true))) return rc;
-  if ((rc = dev_alloc(c, s.parts, parts * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.bins, (size_t)bins_cap * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.ckpt_base, (size_t)std::max<int64_t>(2 * T, 1) * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.ckpt, ckpt_entries(p.npixels) * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.ckpt_ctr, kCkptSegs * kCtrStride * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.heavy_keys, (size_t)heavy_slots * npt, true))) return rc;
-  if ((rc = dev_alloc(c, s.scan_status, (size_t)std::max(scb, 1) * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.counters, kNumCounters * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.block_stats, 2 * kStatSlots * 8, true))) return rc;
-  if ((rc = dev_alloc(c, s.tables, p.layout.bytes, true))) return rc;
-  if ((rc = dev_alloc(c, s.vrec, (size_t)p.total_verts * kVrecFloats * 4, true))) return rc;
-  if ((rc = dev_alloc(c, s.win, (size_t)p.npixels * 4, true))) return rc;
+  for (int k = 0; k < kScratchBufs; k++)
+    if ((rc = dev_alloc(c, *bufs[k], bytes[k], true))) return rc;
   if ((rc = host_alloc(s.h_tables, p.layout.bytes))) return rc;
   if ((rc = host_alloc(s.h_counters, kNumCounters * 8))) return rc;
   return MR_OK;
@@ -594,6 +620,78 @@ int copy_outputs(mr_ctx* c, Slot& s, const BatchPlan& p, const mr_outputs* outs,
   return MR_OK;
 }
 
+// ---------------------------------------------------------------------------
+// mr_render without allocating: every scratch buffer is carved from the
+// static device array of kernels.hip (kNoAllocScratchBytes per device), the
+// batch tables and counters pass through fixed host arrays per device, and
+// outputs must be device buffers.  Stale contents of the array from earlier
+// calls do not matter: the clear kernel and the geometry kernel zero what
+// must start at zero (counters, look-back words, tile counters, checkpoint
+// segment counters), and every other buffer is written before it is read.
+constexpr size_t kNoAllocTableBytes = (size_t)1 << 20;
+static char g_na_tables[kMaxDevices][kNoAllocTableBytes];
+static unsigned long long g_na_counters[kMaxDevices][kNumCounters];
+
+static int render_noalloc(mr_ctx* c, const mr_frame* frame, const mr_outputs* out) {
+  if (!out->on_device) return MR_ERR_ARG;
+  if (c->device < 0 || c->device >= kMaxDevices) return MR_ERR_ARG;
+  char* base = (char*)noalloc_scratch(c->device);
+  if (!base) return MR_ERR_HIP;
+  BatchPlan p;
+  int rc;
+  if ((rc = build_batch(c, frame, 1, p))) return rc;
+  if (p.layout.bytes > kNoAllocTableBytes) return MR_ERR_LIMIT;
+  JobDesc& J = p.jobs[0];
+  J.rgb8 = out->rgb8; J.depth = out->depth; J.winner = out->winner; J.rgb = out->rgb;
+  J.normals = out->normals; J.points = out->points;
+  // Bin list and merge slots sized as the allocating path sizes them first;
+  // they cannot grow here, so an overflow is an error.
+  const uint64_t bins_cap = std::max<uint64_t>(
+      (uint64_t)((double)p.total_tris * c->bin_factor) + (uint64_t)p.ntiles, 1u << 16);
+  const uint32_t heavy = (uint32_t)std::min<uint64_t>(bins_cap / c->frag_size + 16, 1u << 20);
+  size_t bytes[kScratchBufs];
+  DevBuf* bufs[kScratchBufs];
+  Slot ns;
+  scratch_sizes(c, p, bins_cap, heavy, bytes);
+  scratch_bufs(ns.sc, bufs);
+  size_t off = 0;
+  for (int k = 0; k < kScratchBufs; k++) {
+    const size_t b = (bytes[k] + 255) & ~(size_t)255;
+    if (b > kNoAllocScratchBytes - off) return MR_ERR_LIMIT;
+    bufs[k]->p = base + off;
+    bufs[k]->bytes = bytes[k];
+    off += b;
+  }
+  J.win = (uint32_t*)ns.sc.win.p;
+  ns.stream = c->main.stream;
+  char* ht = g_na_tables[c->device];
+  memcpy(ht + p.layout.jobs, p.jobs.data(), p.jobs.size() * sizeof(JobDesc));
+  memcpy(ht + p.layout.bases, p.job_bases.data(), p.job_bases.size() * 4);
+  if (!p.draws.empty()) {
+    memcpy(ht + p.layout.draws, p.draws.data(), p.draws.size() * sizeof(DrawDesc));
+    memcpy(ht + p.layout.draw_begin, p.draw_begin.data(), p.draws.size() * 8);
+    memcpy(ht + p.layout.vg_begin, p.vg_begin.data(), p.vg_begin.size() * 8);
+    memcpy(ht + p.layout.vg_draw, p.vg_draw.data(), p.vg_draw.size() * 4);
+  }
+  memcpy(ht + p.layout.hints, p.hints.data(), p.hints.size() * 8);
+  const BatchArgs a = make_args(c, ns, p);
+  unsigned long long* hc = g_na_counters[c->device];
+  HIP_TRY(hipMemcpyAsync(ns.sc.tables.p, ht, p.layout.bytes, hipMemcpyHostToDevice, ns.stream));
+  if ((rc = launch_stages(a, ns, false, true))) return rc;
+  HIP_TRY(hipMemcpyAsync(hc, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost, ns.stream));
+  HIP_TRY(hipStreamSynchronize(ns.stream));
+  if (hc[kError]) return MR_ERR_DEVICE;
+  if (hc[kBinOverflow] || hc[kHeavyOverflow]) return MR_ERR_NOMEM;
+  mr_stats& st = c->stats;
+  st.triangles_in = p.total_tris;
+  st.triangles_clipped = (int64_t)hc[kClipped];
+  st.records = (int64_t)hc[kItems];
+  st.bin_entries = (int64_t)hc[kBinEntries];
+  st.tiles = p.ntiles;
+  st.retries = 0;
+  return MR_OK;
+}
+
 int render_frames(mr_ctx* c, const mr_frame* frames, int n, const mr_outputs* outs) {
   BatchPlan p;
   std::vector<size_t> offs;
@@ -642,6 +740,9 @@ int mr_ctx_create(int32_t device, uint64_t hbm_budget, mr_ctx** out) {
     delete c;
     return MR_ERR_HIP;
   }
+  // Address of this device's copy of the no-alloc scratch (once per device;
+  // on failure mr_render_noalloc reports MR_ERR_HIP).
+  (void)noalloc_scratch_init(device);
   int cus = 0;
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
   c->num_cus = cus > 0 ? cus : 256;
@@ -893,6 +994,13 @@ int mr_render(mr_ctx* c, const mr_frame* frame, const mr_outputs* out) {
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   HIP_TRY(hipSetDevice(c->device));
   return render_frames(c, frame, 1, out);
+}
+
+int mr_render_noalloc(mr_ctx* c, const mr_frame* frame, const mr_outputs* out) {
+  if (!c || !frame || !out) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  return render_noalloc(c, frame, out);
 }
 
 int mr_render_batch(mr_ctx* c, const mr_frame* frames, int32_t n, const mr_outputs* outs) {

@@ -37,6 +37,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 #include "exact.h"
@@ -1176,6 +1177,45 @@ __global__ __launch_bounds__(kShadeBlock, 7) void k_shade(BatchArgs a) {
   if (!one_draw) __syncthreads();
   if (blockIdx.x == 0) reduce_stats(a, tid);
   if (inside) shade_pixel(a, J, dcache, ncached, id, c, row);
+}
+
+// ---- scratch of the entry point that allocates nothing ----------------------
+//
+// mr_render_noalloc takes every scratch buffer of a frame (items, bins,
+// checkpoints, tile counters, vertex records, winner ids, tables) from this
+// one array instead of allocating.  Each device the library runs on holds its
+// own copy for the life of the process.  Host code never passes `scratch`
+// itself to a kernel or copy: it uses the device address that
+// hipGetSymbolAddress returns for the current device, kept per device below.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return -1;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != device) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  g_scratch_ptr[device] = ptr;
+  return 0;
+}
+
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
 }
 
 // ---- launchers -------------------------------------------------------------

@@ -105,6 +105,14 @@ hipError_t launch_tile(const BatchArgs& a, hipStream_t s);
 // Geometry workgroups one CU holds at once (from the kernel's resources).
 int geom_blocks_per_cu();
 hipError_t launch_shade(const BatchArgs& a, hipStream_t s);
+// The no-alloc entry point's scratch: kNoAllocScratchBytes of device memory
+// per device (a static array).  noalloc_scratch_init(device), with that
+// device current, looks up its address once (0, or -1 on failure);
+// noalloc_scratch(device) returns it (null before the init).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+int noalloc_scratch_init(int device);
+void* noalloc_scratch(int device);
 bool tile_supported(int tile_w, int tile_h);
 // Raster workgroup sizes built per tile size: 8x8 {64, 128}, 16x16 and
 // 32x32 {64, 128, 256, 512}; the default is 64 for 8x8, else 256.
