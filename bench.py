@@ -49,9 +49,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mtris/s + frames/s at 1080p; p50/p99 ms/request, batched vs plain GPU"
 HBM_PEAK_GBS = 8000.0
-STAGES = ["clear", "geometry", "scan", "bin", "raster", "shade"]
+STAGES = ["clear", "geometry", "scan", "bin", "rows", "raster", "shade"]
 KERNEL_OF = {"clear": "k_clear", "geometry": "k_vertex+k_geom", "scan": "k_scan", "bin": "k_bin",
-             "raster": "k_tile", "shade": "k_shade"}
+             "rows": "k_rows", "raster": "k_tile", "shade": "k_shade"}
 
 
 def parse():
@@ -62,6 +62,11 @@ def parse():
     p.add_argument("--workload", default="terrain",
                    choices=["terrain", "megatri", "sphere", "textured"])
     p.add_argument("--tile", type=int, default=0, help="0 = autotune per workload")
+    p.add_argument("--raster-threads", type=int, default=0,
+                   help="with --tile: raster workgroup size (0: default)")
+    p.add_argument("--frag-size", type=int, default=0, help="with --tile: split size (0: default)")
+    p.add_argument("--band-work", type=int, default=-1,
+                   help="row-band split threshold of the tile raster (-1: library default)")
     p.add_argument("--tune-cache", default=None,
                    help="autotuner cache file: reuse (or record) the launch configuration")
     p.add_argument("--no-cpu", action="store_true")
@@ -154,6 +159,7 @@ def kernel_alg_bytes(frame):
         "geometry": pos + idx,  # vertex kernel (positions) + geometry kernel (indices)
         "scan": 0,
         "bin": 0,               # items and pairs are implementation bytes
+        "rows": 0,              # edge checkpoints are implementation bytes
         "raster": 0,            # reads only the bin list (implementation bytes)
         "shade": npx * (3 + 4) + attrs + texel_bytes(frame, None),  # outputs, normals, uvs, texels
     }
@@ -199,6 +205,8 @@ def plain_section(args, torch, mra, frame, wl_name):
         if rc:
             raise RuntimeError(f"mr_render failed: {rc}")
 
+    if args.band_work >= 0:
+        ctx.set_band_work(args.band_work)
     if args.tile:
         ctx.set_tile(args.tile, args.tile)
         tune = {"tile": args.tile, "fixed": True}
@@ -222,12 +230,12 @@ def plain_section(args, torch, mra, frame, wl_name):
     # Per-kernel times from a separate pass with HIP events around each stage
     # (on the context's stream; eager launches).
     ctx.set_profiling(True)
-    stage = np.zeros(6)
+    stage = np.zeros(len(STAGES))
     fms = []
     for _ in range(args.steps):
         step()
         s = ctx.stats()
-        stage += np.array(s.stage_ms[:6])
+        stage += np.array(s.stage_ms[:len(STAGES)])
         fms.append(s.frame_ms)
     ctx.set_profiling(False)
     stage /= args.steps
@@ -238,7 +246,7 @@ def plain_section(args, torch, mra, frame, wl_name):
         gbs = alg[name] / (ms * 1e-3) / 1e9 if ms > 0 and alg[name] else 0.0
         kernels[KERNEL_OF[name]] = {"ms": round(ms, 5), "alg_bytes": alg[name],
                                     "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
-    dom = max(range(1, 6), key=lambda i: stage[i])
+    dom = max(range(1, len(STAGES)), key=lambda i: stage[i])
     dname = STAGES[dom]
     dms = float(stage[dom])
     fb = frame_alg_bytes(frame)
@@ -255,7 +263,7 @@ def plain_section(args, torch, mra, frame, wl_name):
     frame_traffic = (sum(ktraffic.values()) if all(v is not None for v in ktraffic.values())
                      else None)
     achieved = fb / (fmed * 1e-3) / 1e9 if fmed > 0 else 0.0
-    roof = {"bound": "hbm", "kernel": "frame (k_vertex, k_geom, k_scan, k_bin, k_tile, k_shade)",
+    roof = {"bound": "hbm", "kernel": "frame (k_vertex, k_geom, k_scan, k_bin, k_rows, k_tile, k_shade)",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "alg_bytes_per_launch": fb, "avg_launch_ms": round(fmed, 5),
@@ -414,6 +422,8 @@ def serve_batch_profile(args, torch, mra, target=False):
         pool = scenes.serving_pool()
         reqs = scenes.serving_requests(pool, args.serve_batch)
     ctx = mra.Context(0)
+    if args.band_work >= 0:
+        ctx.set_band_work(args.band_work)
     dev = torch.device("cuda:0")
     structs, keep, outs, bufs = [], [], [], []
     for fr in reqs:
@@ -426,7 +436,17 @@ def serve_batch_profile(args, torch, mra, target=False):
     n = len(reqs)
     farr = (mr_frame * n)(*structs)
     oarr = (mr_outputs * n)(*outs)
-    tune = ctx.autotune(reqs, outs=outs, reps=2, cache_path=args.tune_cache)
+    if args.tile:
+        ctx.set_tile(args.tile, args.tile)
+        if args.raster_threads:
+            ctx.set_raster_threads(args.raster_threads)
+        if args.frag_size:
+            ctx.set_frag_size(args.frag_size)
+        ctx.set_bin_factor(1.5)
+        tune = {"tile": args.tile, "raster_threads": args.raster_threads,
+                "frag_size": args.frag_size, "fixed": True}
+    else:
+        tune = ctx.autotune(reqs, outs=outs, reps=2, cache_path=args.tune_cache)
 
     def batch():
         rc = ctx.lib.mr_render_batch(ctx.h, farr, n, oarr)

@@ -100,7 +100,8 @@ typedef struct mr_stats {
   int64_t tiles;              /* screen tiles                                     */
   int64_t retries;            /* re-runs after a bin-capacity overflow           */
   double stage_ms[8];         /* per-stage GPU time when profiling is on:        */
-                              /* 0 clear, 1 geometry, 2 scan, 3 bin, 4 raster     */
+                              /* 0 clear, 1 geometry, 2 scan, 3 bin, 4 rows      */
+                              /* (edge checkpoints), 5 raster, 6 shade            */
   double frame_ms;            /* first to last kernel of the frame (profiling)  */
 } mr_stats;
 
@@ -153,6 +154,11 @@ int mr_ctx_autotune(mr_ctx* ctx, const mr_frame* frames, int32_t nframes,
  * tile with more is split into parts merged through device memory
  * (64 .. 1<<20, default 1024). */
 int mr_ctx_set_frag_size(mr_ctx* ctx, int32_t pairs);
+/* Tiles whose walk work (bbox pixels of their entries inside the tile, plus 4
+ * per row and 16 per entry) exceeds this are rastered as row bands, one
+ * workgroup per band (0 = never; default 32768).  No counterpart in the
+ * reference: load balance of the tile raster. */
+int mr_ctx_set_band_work(mr_ctx* ctx, int32_t units);
 /* Bin-list capacity in (tile, triangle) entries per input triangle
  * (>= 1); overflow re-runs the frame with a larger list. */
 int mr_ctx_set_bin_factor(mr_ctx* ctx, float entries_per_triangle);

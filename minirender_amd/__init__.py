@@ -139,6 +139,9 @@ class Context:
     def set_frag_size(self, pairs: int) -> None:
         check(self.lib.mr_ctx_set_frag_size(self.h, pairs), "mr_ctx_set_frag_size")
 
+    def set_band_work(self, units: int) -> None:
+        check(self.lib.mr_ctx_set_band_work(self.h, units), "mr_ctx_set_band_work")
+
     def set_bin_factor(self, f: float) -> None:
         check(self.lib.mr_ctx_set_bin_factor(self.h, f), "mr_ctx_set_bin_factor")
 

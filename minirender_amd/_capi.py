@@ -117,6 +117,7 @@ SIGNATURES = {
     "mr_texture_release": (C.c_int, [_P, _I]),
     "mr_render": (C.c_int, [_P, C.POINTER(mr_frame), C.POINTER(mr_outputs)]),
     "mr_render_noalloc": (C.c_int, [_P, C.POINTER(mr_frame), C.POINTER(mr_outputs)]),
+    "mr_ctx_set_band_work": (C.c_int, [_P, _I]),
     "mr_render_batch": (C.c_int, [_P, C.POINTER(mr_frame), _I, C.POINTER(mr_outputs)]),
     "mr_ctx_autotune": (C.c_int, [_P, C.POINTER(mr_frame), _I, C.POINTER(mr_outputs), _I,
                                   C.c_char_p, C.POINTER(mr_tuning)]),

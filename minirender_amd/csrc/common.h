@@ -112,6 +112,7 @@ MR_HD inline int64_t xcd_remap(int64_t b, int64_t n) {
 // Per-tile counters are spread one per 64-byte line: the atomics of the
 // binner then spread over all memory channels instead of a few.
 constexpr int kCtrStride = 16;
+constexpr int kCtrStride64 = kCtrStride / 2;  // the same spacing in 64-bit words
 
 // The edge-checkpoint buffer is cut into this many segments, each with its
 // own allocation counter (kCtrStride words apart): one shared counter would
@@ -132,6 +133,10 @@ constexpr int kScanItems = 8;    // elements per thread in the tile-count scan
 constexpr uint32_t kNoWinner = 0xFFFFFFFFu;
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 constexpr uint32_t kEmptyTile = 0xFFFFFFFEu;  // tile_heavy of a tile with no bin entry
+// A raster work item (parts list entry .y) that covers a band of the tile's
+// rows with all of its entries: bit 31 set, band count in bits 8-15, band
+// index in bits 0-7.  Other entries are entry-range parts (the part index).
+constexpr uint32_t kBandPart = 0x80000000u;
 constexpr float kClearDepth = 1e11f;  // Renderer.cpp:116
 // Job lookup hints: one per kHintSpan consecutive tiles or shading blocks.
 constexpr int kHintShift = 6;

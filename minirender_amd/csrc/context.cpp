@@ -111,7 +111,7 @@ void slot_free(mr_ctx* c, Slot& s) {
   DevBuf* bufs[] = {&x.items, &x.region_count, &x.tile_count, &x.tile_off,
                     &x.tile_cursor, &x.tile_heavy, &x.tile_list, &x.tile_done, &x.parts, &x.bins,
                     &x.ckpt_base, &x.ckpt, &x.ckpt_ctr, &x.heavy_keys, &x.scan_status, &x.counters, &x.block_stats,
-                    &x.tables, &x.vrec, &x.win, &x.outputs};
+                    &x.tables, &x.vrec, &x.win, &x.wide, &x.outputs};
   for (DevBuf* b : bufs) dev_free(c, *b, true);
   HostBuf* hbufs[] = {&x.h_tables, &x.h_counters, &x.h_outputs};
   for (HostBuf* b : hbufs) host_free(*b);
@@ -276,16 +276,20 @@ static size_t ckpt_entries(int64_t npixels) { return (size_t)npixels * 2 + 65536
 
 // Device scratch of a batch, buffer by buffer (bytes), in the order of
 // scratch_bufs().
-constexpr int kScratchBufs = 20;
+constexpr int kScratchBufs = 21;
 
 static void scratch_sizes(mr_ctx* c, const BatchPlan& p, uint64_t bins_cap, uint32_t heavy_slots,
-                          size_t out[kScratchBufs]) {
+                          size_t parts_want, size_t out[kScratchBufs]) {
   const int64_t T = p.total_tris;
   const int scb = scan_blocks_for((int32_t)p.ntiles);
   const size_t nt = (size_t)p.ntiles * 4;
   const int64_t nregions = std::max<int64_t>((T + 63) / 64, 1);
   const size_t npt = (size_t)c->tile_w * c->tile_h * 8;
-  const size_t parts = (size_t)(bins_cap / c->frag_size) + heavy_slots + 16;
+  // entry parts of split tiles, and row bands of heavy tiles (one per tile
+  // on average; a batch that wants more re-runs with the measured need)
+  const size_t parts = std::max<size_t>((size_t)(bins_cap / c->frag_size) + heavy_slots + 16 +
+                                            (size_t)p.ntiles,
+                                        parts_want);
   const size_t v[kScratchBufs] = {
       (size_t)nregions * kRegion * sizeof(BinItem),  // items
       (size_t)nregions * 4,                          // region_count
@@ -307,6 +311,7 @@ static void scratch_sizes(mr_ctx* c, const BatchPlan& p, uint64_t bins_cap, uint
       p.layout.bytes,                                // tables
       (size_t)p.total_verts * kVrecFloats * 4,       // vrec
       (size_t)p.npixels * 4,                         // win
+      (ckpt_entries(p.npixels) / 2 + 64) * 8,        // row tasks of the row pre-pass
   };
   for (int k = 0; k < kScratchBufs; k++) out[k] = v[k];
 }
@@ -316,7 +321,7 @@ static void scratch_bufs(Scratch& x, DevBuf* b[kScratchBufs]) {
                              &x.tile_cursor, &x.tile_heavy, &x.tile_list, &x.tile_done,
                              &x.parts, &x.bins, &x.ckpt_base, &x.ckpt, &x.ckpt_ctr,
                              &x.heavy_keys, &x.scan_status, &x.counters, &x.block_stats,
-                             &x.tables, &x.vrec, &x.win};
+                             &x.tables, &x.vrec, &x.win, &x.wide};
   for (int k = 0; k < kScratchBufs; k++) b[k] = v[k];
 }
 
@@ -325,7 +330,7 @@ static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins
   Scratch& s = sl.sc;
   size_t bytes[kScratchBufs];
   DevBuf* bufs[kScratchBufs];
-  scratch_sizes(c, p, bins_cap, heavy_slots, bytes);
+  scratch_sizes(c, p, bins_cap, heavy_slots, sl.parts_want, bytes);
   scratch_bufs(s, bufs);
   int rc;
   for (int k = 0; k < kScratchBufs; k++)
@@ -342,24 +347,38 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.items = (BinItem*)s.items.p;
   a.region_count = (uint32_t*)s.region_count.p;
   a.nregions = (p.total_tris + 63) / 64;
-  a.tile_count = (uint32_t*)s.tile_count.p;
+  a.tile_count = (unsigned long long*)s.tile_count.p;
   a.tile_off = (uint32_t*)s.tile_off.p;
   a.tile_cursor = (uint32_t*)s.tile_cursor.p;
   a.tile_heavy = (uint32_t*)s.tile_heavy.p;
   a.tile_list = (uint4*)s.tile_list.p;
   a.tile_done = (uint32_t*)s.tile_done.p;
   a.parts = (uint2*)s.parts.p;
-  a.parts_cap = (uint32_t)std::min<size_t>(s.parts.bytes / 8, 0xFFFFFFF0u);
+  // Raster work items of split tiles and row bands this batch may use: its
+  // own bound, not the buffer's capacity (which only grows, e.g. over an
+  // autotune's candidates): the raster grid is ntiles + parts_cap, and a
+  // batch that wants more re-runs with the measured need.
+  const uint64_t batch_bins = (uint64_t)((double)p.total_tris * c->bin_factor) + (uint64_t)p.ntiles;
+  const uint64_t batch_parts = std::max<uint64_t>(
+      batch_bins / c->frag_size + (uint64_t)sl.heavy_slots + 16 + 2 * (uint64_t)p.ntiles,
+      (uint64_t)sl.parts_want);
+  a.parts_cap = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(s.parts.bytes / 8, batch_parts),
+                                             0xFFFFFFF0u);
   a.bins = (uint2*)s.bins.p;
   a.bins_cap = s.bins.bytes / 8;
   if (a.bins_cap > 0xFFFFFFF0ull) a.bins_cap = 0xFFFFFFF0ull;
   a.ckpt_base = (int2*)s.ckpt_base.p;
   a.ckpt = (float2*)s.ckpt.p;
-  a.ckpt_cap = std::min<uint64_t>(s.ckpt.bytes / 8, 0xFFFFFFF0ull);
+  // Checkpoint capacity, held to twice the row-task entries: the row tasks
+  // of every item given checkpoints then fit in their segment.
+  a.ckpt_cap = std::min<uint64_t>(std::min<uint64_t>(s.ckpt.bytes / 8, 2 * (s.wide.bytes / 8)),
+                                  0xFFFFFFF0ull);
   a.ckpt_ctr = (unsigned long long*)s.ckpt_ctr.p;
+  a.ckpt_rows = (unsigned long long*)s.wide.p;
   a.heavy_keys = (unsigned long long*)s.heavy_keys.p;
   a.heavy_cap = (uint32_t)(s.heavy_keys.bytes / ((size_t)c->tile_w * c->tile_h * 8));
   a.frag_size = c->frag_size;
+  a.band_work = c->band_work;
   a.scan_status = (unsigned long long*)s.scan_status.p;
   a.counters = (unsigned long long*)s.counters.p;
   a.block_stats = (unsigned long long*)s.block_stats.p;
@@ -403,10 +422,12 @@ static int launch_stages(const BatchArgs& a, Slot& s, bool profile, bool clear) 
   if (profile) HIP_TRY(hipEventRecord(s.ev[3], st));
   HIP_TRY(launch_bin(a, st));
   if (profile) HIP_TRY(hipEventRecord(s.ev[4], st));
-  HIP_TRY(launch_tile(a, st));
+  HIP_TRY(launch_rows(a, st));
   if (profile) HIP_TRY(hipEventRecord(s.ev[5], st));
-  HIP_TRY(launch_shade(a, st));
+  HIP_TRY(launch_tile(a, st));
   if (profile) HIP_TRY(hipEventRecord(s.ev[6], st));
+  HIP_TRY(launch_shade(a, st));
+  if (profile) HIP_TRY(hipEventRecord(s.ev[7], st));
   return MR_OK;
 }
 
@@ -533,6 +554,7 @@ int finish_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
       if ((rc = dev_alloc(c, x.bins, (size_t)need * 8, true))) return rc;
     }
     if (hc[kHeavyOverflow]) {
+      s.parts_want = std::max<size_t>(s.parts_want, (size_t)(hc[kParts] + hc[kParts] / 4 + 16));
       s.heavy_slots = (uint32_t)(hc[kHeavy] + hc[kHeavy] / 4 + 16);
       if ((rc = dev_alloc(c, x.heavy_keys,
                           (size_t)s.heavy_slots * c->tile_w * c->tile_h * 8, true)))
@@ -551,11 +573,11 @@ int finish_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
     st.retries = s.retries;
     if (profile) {
       float ms = 0;
-      for (int i = 0; i < 6; i++) {
+      for (int i = 0; i < kStages; i++) {
         hipEventElapsedTime(&ms, s.ev[i], s.ev[i + 1]);
         st.stage_ms[i] = ms;
       }
-      hipEventElapsedTime(&ms, s.ev[0], s.ev[6]);
+      hipEventElapsedTime(&ms, s.ev[0], s.ev[kStages]);
       st.frame_ms = ms;
     }
   }
@@ -652,7 +674,7 @@ static int render_noalloc(mr_ctx* c, const mr_frame* frame, const mr_outputs* ou
   size_t bytes[kScratchBufs];
   DevBuf* bufs[kScratchBufs];
   Slot ns;
-  scratch_sizes(c, p, bins_cap, heavy, bytes);
+  scratch_sizes(c, p, bins_cap, heavy, 0, bytes);
   scratch_bufs(ns.sc, bufs);
   size_t off = 0;
   for (int k = 0; k < kScratchBufs; k++) {
@@ -805,6 +827,13 @@ int mr_ctx_set_raster_threads(mr_ctx* c, int32_t threads) {
   if (threads == 0) threads = default_raster_threads(c->tile_w);
   if (!raster_threads_supported(c->tile_w, threads)) return MR_ERR_ARG;
   c->raster_threads = threads;
+  return MR_OK;
+}
+
+int mr_ctx_set_band_work(mr_ctx* c, int32_t units) {
+  if (!c || units < 0) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  c->band_work = (uint32_t)units;
   return MR_OK;
 }
 

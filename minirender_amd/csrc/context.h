@@ -14,6 +14,9 @@ namespace mr {
 
 constexpr size_t kGraphCache = 32;
 constexpr int kOutKinds = 6;  // rgb8, depth, winner, rgb, normals, points (mr_outputs order)
+// Stages timed when profiling (mr_stats.stage_ms order): clear, geometry
+// (vertex + geometry kernels), scan, bin, rows, raster, shade.
+constexpr int kStages = 7;
 
 struct DevBuf {
   void* p = nullptr;
@@ -51,7 +54,7 @@ struct TexRec {
 struct Scratch {
   DevBuf items, region_count, tile_count, tile_off, tile_cursor, tile_heavy, tile_list,
       tile_done, parts, bins, ckpt_base, ckpt, ckpt_ctr, heavy_keys, scan_status, counters,
-      block_stats, tables, vrec, win, outputs;
+      block_stats, tables, vrec, win, wide, outputs;
   // The batch tables (jobs, job tile/shade bases, draws, draw starts) sit in
   // one pinned staging buffer and one device buffer, uploaded by one copy.
   HostBuf h_tables, h_counters, h_outputs;
@@ -89,9 +92,10 @@ struct Slot {
   Scratch sc;
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
-  hipEvent_t ev[7] = {};  // stage boundaries (profiling)
+  hipEvent_t ev[kStages + 1] = {};  // stage boundaries (profiling)
   hipEvent_t done = nullptr;
   uint32_t heavy_slots = 0;  // merge buffers of split tiles (grown on demand)
+  size_t parts_want = 0;     // raster work items a batch needed (after an overflow)
   int32_t last_geom_grid = 0, last_bin_grid = 0;
   int64_t retries = 0;
   // Steady state: batches' uploads + stages + counter read-back captured as
@@ -123,6 +127,7 @@ struct mr_ctx {
   int32_t raster_threads = 256;  // tile raster workgroup size
   float bin_factor = 3.0f;
   uint32_t frag_size = 1024;  // max (tile, triangle) pairs per raster item
+  uint32_t band_work = 32768;  // walk work above which a tile is rastered in row bands
   bool profiling = false;
   bool graphs = false;  // capture steady-state batches as HIP graphs
   mr_stats stats{};

@@ -93,7 +93,7 @@ __global__ void k_clear(BatchArgs a) {
     if (i < kNumCounters) a.counters[i] = 0;
     if (i < a.scan_blocks) a.scan_status[i] = 0;
     if (i < a.ntiles) {
-      a.tile_count[(size_t)i * kCtrStride] = 0;
+      a.tile_count[(size_t)i * kCtrStride64] = 0;
       a.tile_done[i] = 0;
     }
   }
@@ -254,6 +254,29 @@ __device__ inline int tri_setup(const BatchArgs& a, const DrawDesc& D, const Job
   return 0;
 }
 
+// Whether a (sub-)triangle covers any pixel centre of its bounding box.
+// For a box of at most kCoverProbe x kCoverProbe pixels this is decided with
+// the raster's own arithmetic: each row starts at the box's left column
+// (Renderer.cpp:240-242) and steps e += n.x (:243), the chain the raster
+// continues from there, so a triangle found to cover nothing here covers
+// nothing in the raster either and is not binned.  Most triangles smaller
+// than a pixel cover no centre.  Larger boxes are assumed to cover one.
+constexpr int kCoverProbe = 3;
+__device__ inline bool covers_any(const TriSetup& s) {
+  if (s.bx1 - s.bx0 >= kCoverProbe || s.by1 - s.by0 >= kCoverProbe) return true;
+  for (int row = s.by0; row <= s.by1; row++) {
+    float e1, e2;
+    edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
+               (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
+    for (int c = s.bx0; c <= s.bx1; c++) {
+      if (covered(e1, e2)) return true;
+      e1 = e1 + s.n1x;
+      e2 = e2 + s.n2x;
+    }
+  }
+  return false;
+}
+
 // Setup of sub-triangle `sub` of a binned triangle from its corners.
 __device__ inline void tri_setup_corners(const JobDesc& J, const f3 e[3], const ScreenVtx v[3],
                                          int sub, bool no_clip, TriSetup& s) {
@@ -346,19 +369,70 @@ __global__ __launch_bounds__(256) void k_vertex(BatchArgs a) {
   }
 }
 
-// For every active lane's tile rectangle, visit all (lane, tile) pairs with
-// the whole wavefront (64 pairs per step, owners found by binary search of
-// the wave prefix sum).  Lanes with equal tiles form a group (at most
-// kMaxGroups groups per step; further lanes are groups of one) whose leader
-// calls op(tile, group size) once.  kScatter: op returns the first bin slot
-// of the group and every lane writes its (id, draw) pair behind it.
+// Pairs of one wavefront step: lanes with ok set hold a pair (tile key, id,
+// draw).  Lanes with equal keys form a group (at most kMaxGroups groups per
+// step; further lanes are groups of one) whose leader calls op(key, group
+// size) once.  kScatter: op returns the first bin slot of the group and
+// every lane writes its (id, draw) pair behind it.
+template <bool kScatter, typename Op>
+__device__ inline void wave_pairs_apply(const BatchArgs& a, bool ok, int key, uint32_t id,
+                                        uint32_t draw, uint32_t work, Op&& op) {
+  const int lane = threadIdx.x & 63;
+  // Group equal keys (bounded: steps of a wide triangle have ~64 distinct
+  // tiles, where grouping only costs).
+  u64 pend = __ballot(ok);
+  int leader = lane;
+  uint32_t rank = 0, gsize = 1;
+  constexpr int kMaxGroups = 8;
+  for (int it = 0; it < kMaxGroups && pend; it++) {
+    const int l = __builtin_ctzll(pend);
+    const int kk = __builtin_amdgcn_readlane(key, l);
+    const u64 grp = pend & __ballot(key == kk);
+    if ((grp >> lane) & 1ull) {
+      leader = l;
+      rank = (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
+    }
+    if (lane == l) gsize = (uint32_t)__popcll(grp);
+    pend &= ~grp;
+  }
+  uint32_t old = 0;
+  if (ok && lane == leader) old = op(key, gsize, work * gsize);
+  if (kScatter) {
+    old = __shfl(old, leader);
+    if (ok) {
+      const uint32_t slot = old + rank;
+      if ((u64)slot < a.bins_cap) a.bins[slot] = make_uint2(id, draw);
+      else atomicOr(&a.counters[kBinOverflow], 1ull);
+    }
+  }
+}
+
+// Walk work of a (triangle, tile) pair: the triangle's bbox pixels inside the
+// tile, 4 per bbox row there and 16 for the entry (a rough cost of the
+// raster's setup, row starts and column steps).
+__device__ inline uint32_t pair_work(int rows, int cols) {
+  return (uint32_t)(rows * cols + 4 * rows + 16);
+}
+
+// For every active lane's pixel bbox, visit all (lane, tile) pairs with the
+// whole wavefront (64 pairs per step, owners found by binary search of the
+// wave prefix sum) and hand each step to wave_pairs_apply.  When every
+// active bbox lies in a single tile (small triangles, the common case) the
+// lanes' own tiles are the step.
 template <bool kScatter, typename Op>
 __device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id, uint32_t draw,
-                                int tbase, int tiles_x, int tx0, int tx1,
-                                int ty0, int ty1, Op&& op) {
+                                int tbase, int tiles_x, int bx0, int bx1, int by0, int by1,
+                                int sw, int sh, Op&& op) {
   const int lane = threadIdx.x & 63;
-  if (!active) { tx0 = tx1 = ty0 = ty1 = 0; }
+  if (!active) { bx0 = bx1 = by0 = by1 = 0; }
+  const int tx0 = bx0 >> sw, tx1 = bx1 >> sw, ty0 = by0 >> sh, ty1 = by1 >> sh;
   const int cnt = active ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0;
+  if (__ballot(cnt > 1) == 0ull) {
+    const bool ok = cnt == 1;
+    const uint32_t work = kScatter ? 0u : pair_work(by1 - by0 + 1, bx1 - bx0 + 1);
+    wave_pairs_apply<kScatter>(a, ok, ok ? tbase + ty0 * tiles_x + tx0 : -1, id, draw, work, op);
+    return;
+  }
   int P = cnt;  // inclusive prefix over the wave
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -381,41 +455,26 @@ __device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id, ui
     const int ob = __shfl(tbase, lo), otx = __shfl(tiles_x, lo);
     const int w = max(ox1 - ox0 + 1, 1);
     const int r = k / w;
-    const int key = ok ? ob + (oy0 + r) * otx + (ox0 + (k - r * w)) : -1;
-    // Group equal keys (bounded: steps of a wide triangle have ~64 distinct
-    // tiles, where grouping only costs).
-    u64 pend = __ballot(ok);
-    int leader = lane;
-    uint32_t rank = 0, gsize = 1;
-    constexpr int kMaxGroups = 8;
-    for (int it = 0; it < kMaxGroups && pend; it++) {
-      const int l = __builtin_ctzll(pend);
-      const int kk = __builtin_amdgcn_readlane(key, l);
-      const u64 grp = pend & __ballot(key == kk);
-      if ((grp >> lane) & 1ull) {
-        leader = l;
-        rank = (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
-      }
-      if (lane == l) gsize = (uint32_t)__popcll(grp);
-      pend &= ~grp;
-    }
-    uint32_t old = 0;
-    if (ok && lane == leader) old = op(key, gsize);
+    const int ptx = ox0 + (k - r * w), pty = oy0 + r;
+    const int key = ok ? ob + pty * otx + ptx : -1;
+    uint32_t oid = 0, odraw = 0, work = 0;
     if (kScatter) {
-      const uint32_t oid = __shfl(id, lo), odraw = __shfl(draw, lo);
-      old = __shfl(old, leader);
-      if (ok) {
-        const uint32_t slot = old + rank;
-        if ((u64)slot < a.bins_cap) a.bins[slot] = make_uint2(oid, odraw);
-        else atomicOr(&a.counters[kBinOverflow], 1ull);
-      }
+      oid = __shfl(id, lo);
+      odraw = __shfl(draw, lo);
+    } else {
+      const int obx0 = __shfl(bx0, lo), obx1 = __shfl(bx1, lo);
+      const int oby0 = __shfl(by0, lo), oby1 = __shfl(by1, lo);
+      const int rows = min(oby1, ((pty + 1) << sh) - 1) - max(oby0, pty << sh) + 1;
+      const int cols = min(obx1, ((ptx + 1) << sw) - 1) - max(obx0, ptx << sw) + 1;
+      work = pair_work(rows, cols);
     }
+    wave_pairs_apply<kScatter>(a, ok, key, oid, odraw, work, op);
   }
 }
 
 // (a)+(b): one lane per input triangle; wavefront w writes its survivors to
 // items[w * kRegion ...] and their number to region_count[w].
-__global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
+__global__ __launch_bounds__(256, 7) void k_geom(BatchArgs a) {
   // The batch counters and checkpoint segment counters are first used by the
   // scan and the binner: clear them here.
   if (blockIdx.x == 0 && threadIdx.x < kNumCounters) a.counters[threadIdx.x] = 0;
@@ -466,6 +525,9 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
         setup(a.draws[d], a.jobs[a.draws[d].job]);
       }
       draw = (uint32_t)d;
+      // A triangle that covers no pixel centre is not binned (clipped ones,
+      // rare, are binned as they are).
+      if (!clipped && ns == 1 && !covers_any(s0)) ns = 0;
     }
     nclipped += (uint32_t)__popcll(__ballot(clipped));
     const u64 b1 = __ballot(ns >= 1), b2 = __ballot(ns == 2);
@@ -481,15 +543,16 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
     if (ns == 2)
       a.items[pos + 1] = BinItem{id1, draw, (uint16_t)s1.bx0, (uint16_t)s1.bx1,
                                  (uint16_t)s1.by0, (uint16_t)s1.by1};
-    auto count = [&](int key, uint32_t n) -> uint32_t {
-      atomicAdd(&a.tile_count[(size_t)key * kCtrStride], n);
+    // Pairs and walk work per tile, in one 64-bit atomic per group.
+    auto count = [&](int key, uint32_t n, uint32_t work) -> uint32_t {
+      atomicAdd(&a.tile_count[(size_t)key * kCtrStride64], ((u64)work << 32) | (u64)n);
       return 0u;
     };
-    wave_tile_pairs<false>(a, ns >= 1, id0, draw, tbase, tiles_x, s0.bx0 >> sw,
-                           s0.bx1 >> sw, s0.by0 >> sh, s0.by1 >> sh, count);
+    wave_tile_pairs<false>(a, ns >= 1, id0, draw, tbase, tiles_x, s0.bx0, s0.bx1, s0.by0,
+                           s0.by1, sw, sh, count);
     if (b2)
-      wave_tile_pairs<false>(a, ns == 2, id1, draw, tbase, tiles_x, s1.bx0 >> sw,
-                             s1.bx1 >> sw, s1.by0 >> sh, s1.by1 >> sh, count);
+      wave_tile_pairs<false>(a, ns == 2, id1, draw, tbase, tiles_x, s1.bx0, s1.bx1, s1.by0,
+                             s1.by1, sw, sh, count);
   }
   __shared__ uint32_t sh_clipped;
   if (threadIdx.x == 0) sh_clipped = 0;
@@ -499,9 +562,26 @@ __global__ __launch_bounds__(256) void k_geom(BatchArgs a) {
   if (threadIdx.x == 0 && blockIdx.x < kStatSlots) a.block_stats[blockIdx.x] = sh_clipped;
 }
 
+// Row bands of a tile rastered by several workgroups (each with all of the
+// tile's entries, rows of its band only): tiles with at most frag_size pairs
+// but more walk work than band_work, about band_work per band, whose entries
+// walk kBandMinEntryWork or more on average (every band sets up every entry:
+// tiles of many small triangles are split by entries instead).  0: none.
+constexpr uint32_t kBandMinEntryWork = 64;
+__device__ inline uint32_t band_count(const BatchArgs& a, uint32_t cnt, uint32_t work) {
+  if (cnt == 0 || cnt > a.frag_size || a.band_work == 0 || work <= a.band_work) return 0;
+  if (work < kBandMinEntryWork * cnt) return 0;
+  uint32_t nb = (work + a.band_work - 1) / a.band_work;
+  if (nb > (uint32_t)a.tile_h) nb = (uint32_t)a.tile_h;
+  const uint32_t bh = ((uint32_t)a.tile_h + nb - 1) / nb;
+  nb = ((uint32_t)a.tile_h + bh - 1) / bh;
+  return nb >= 2 ? nb : 0;
+}
+
 // (c) exclusive scan of (tile, triangle) pairs per tile -> offsets and
 // cursors; tiles with more than frag_size pairs get a merge slot and their
-// parts are listed for the raster.
+// parts are listed for the raster, tiles with much walk work their row
+// bands.
 __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
   __shared__ int sh_blk;
   __shared__ u64 sh_w[kScanBlock / 64];
@@ -512,11 +592,13 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
   __syncthreads();
   const int blk = sh_blk;
   const int base = blk * (kScanBlock * kScanItems) + tid * kScanItems;
-  uint32_t v[kScanItems];
+  uint32_t v[kScanItems], nbv[kScanItems];
   u64 sum = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    v[i] = (base + i < a.ntiles) ? a.tile_count[(size_t)(base + i) * kCtrStride] : 0u;
+    const u64 tc = (base + i < a.ntiles) ? a.tile_count[(size_t)(base + i) * kCtrStride64] : 0ull;
+    v[i] = (uint32_t)tc;
+    nbv[i] = band_count(a, v[i], (uint32_t)(tc >> 32));
     sum += v[i];
   }
   u64 x = sum;  // wave inclusive scan
@@ -548,6 +630,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
     if (base + i < a.ntiles && v[i] > a.frag_size) {
       nh++;
       npt += (v[i] + a.frag_size - 1) / a.frag_size;
+    } else if (base + i < a.ntiles && nbv[i]) {
+      npt += nbv[i];
     } else if (base + i < a.ntiles && v[i] > 0) {
       nl++;
     }
@@ -613,6 +697,14 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
         }
         hs++;
         ps += nparts;
+      } else if (nbv[i]) {
+        if (ps + nbv[i] <= a.parts_cap) {
+          for (uint32_t p = 0; p < nbv[i]; p++)
+            a.parts[ps + p] = make_uint2((uint32_t)t, kBandPart | (nbv[i] << 8) | p);
+        } else {
+          atomicOr(&a.counters[kHeavyOverflow], 1ull);
+        }
+        ps += nbv[i];
       } else if (v[i] > 0) {
         a.tile_list[ls++] = make_uint4((uint32_t)t, o, v[i], 0u);
       } else {
@@ -624,35 +716,82 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
   }
 }
 
-// Edge checkpoints of the wavefront's items: an item spanning ntx > 1 tile
-// columns gets ntx entries per pixel row (one per column, where the row
-// starts in it), taken by one atomic per wavefront from the checkpoint
-// segment of the wavefront (wavefronts spread over kCkptSegs segments).
-// Its base is stored pre-offset by its first row and column, so that a
-// reader needs only the row and the tile column.  Items without checkpoints
-// (one tile column, or no room left) get {0, 0}.
-__device__ inline void ckpt_alloc(const BatchArgs& a, bool act, const BinItem& it, int sw) {
+// Edge checkpoints of the wavefront's items: an item at least
+// kCkptMinSpan pixels wide that spans ntx > 1 tile columns gets ntx entries
+// per pixel row (one per column, where the row enters it) and one row task
+// per pixel row for the row pre-pass, all taken from the checkpoint segment
+// of the wavefront (wavefronts spread over kCkptSegs segments) by one 64-bit
+// atomic: entries in the low 40 bits of the segment counter, row tasks in
+// the high 24.  Every such row holds at least two entries, so a segment's
+// row tasks fit in half its entry count.  The base is stored pre-offset by
+// the item's first row and column, so that a reader needs only the row and
+// the tile column.  Items without checkpoints (narrower, or no room left)
+// get {0, 0}: the raster and shading walk their rows from the bbox's left
+// column (fewer than kCkptMinSpan steps beyond a tile).
+constexpr int kCkptMinSpan = 64;
+constexpr u64 kCkptEntryMask = (1ull << 40) - 1;
+
+__device__ inline void ckpt_alloc(const BatchArgs& a, bool act, const BinItem& it, int sw,
+                                  uint32_t item) {
   const int lane = threadIdx.x & 63;
   const int btx = it.bx0 >> sw;
   const int ntx = act ? (it.bx1 >> sw) - btx + 1 : 0;
-  const u64 need = ntx > 1 ? (u64)ntx * (u64)(it.by1 - it.by0 + 1) : 0ull;
-  u64 P = need;  // inclusive prefix over the wave
+  const int rows = it.by1 - it.by0 + 1;
+  const bool want = ntx > 1 && it.bx1 - it.bx0 + 1 >= kCkptMinSpan;
+  const u64 need = want ? (u64)ntx * (u64)rows : 0ull;
+  const uint32_t nrow = want ? (uint32_t)rows : 0u;
+  u64 P = need;  // inclusive prefixes over the wave
+  uint32_t R = nrow;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
     const u64 y = __shfl_up(P, o);
-    if (lane >= o) P += y;
+    const uint32_t yr = __shfl_up(R, o);
+    if (lane >= o) {
+      P += y;
+      R += yr;
+    }
   }
   const u64 total = __shfl(P, 63);
+  if (total == 0ull) {
+    if (act) a.ckpt_base[it.id] = make_int2(0, 0);
+    return;
+  }
+  const uint32_t rtotal = __shfl(R, 63);
   const u64 seg_cap = a.ckpt_cap / kCkptSegs;
+  const u64 seg_rows = seg_cap / 2;
   const int seg = (int)((blockIdx.x * 4u + (threadIdx.x >> 6)) % kCkptSegs);
-  u64 base = 0;
-  if (total && lane == 63) base = atomicAdd(&a.ckpt_ctr[seg * kCtrStride], total);
-  base = __shfl(base, 63);
+  u64 old = 0;
+  if (lane == 63) old = atomicAdd(&a.ckpt_ctr[seg * kCtrStride], ((u64)rtotal << 40) | total);
+  old = __shfl(old, 63);
+  const u64 base = old & kCkptEntryMask, rbase = old >> 40;
+  bool ok = false;
   if (act) {
     const u64 first = base + P - need;  // within the segment
     const int64_t off = (int64_t)(seg * seg_cap + first) - (int64_t)it.by0 * ntx - btx;
-    const bool ok = need && first + need <= seg_cap && off >= INT32_MIN && off <= INT32_MAX;
+    ok = need && first + need <= seg_cap && off >= INT32_MIN && off <= INT32_MAX;
     a.ckpt_base[it.id] = ok ? make_int2((int)off, ntx) : make_int2(0, 0);
+  }
+  // Row tasks of the items that got checkpoints: (item << 32) | row, the
+  // wavefront's rows spread over its lanes (owners by binary search of the
+  // prefix).  An item that got no room reserved its tasks all the same:
+  // they are written as "no task" (~0) and skipped.
+  const uint32_t ex = R - nrow;
+  for (uint32_t j = 0; j < rtotal; j += 64) {
+    const uint32_t q = j + lane;
+    int lo = 0, hi = 63;
+#pragma unroll
+    for (int st = 0; st < 6; st++) {
+      const int mid = (lo + hi) >> 1;
+      if (__shfl(R, mid) > q) hi = mid; else lo = mid + 1;
+    }
+    const uint32_t oex = __shfl(ex, lo);
+    const int oby0 = __shfl((int)it.by0, lo);
+    const uint32_t oitem = __shfl(item, lo);
+    const bool ook = __shfl((int)ok, lo) != 0;
+    const u64 slot = rbase + q;
+    if (q < rtotal && slot < seg_rows)
+      a.ckpt_rows[(u64)seg * seg_rows + slot] =
+          ook ? ((u64)oitem << 32) | (uint32_t)(oby0 + (int)(q - oex)) : ~0ull;
   }
 }
 
@@ -668,7 +807,7 @@ __global__ __launch_bounds__(256, 8) void k_bin(BatchArgs a) {
   const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
   const int64_t nw = (int64_t)gridDim.x * 4;
   u64 nitems = 0;
-  auto reserve = [&](int key, uint32_t n) -> uint32_t {
+  auto reserve = [&](int key, uint32_t n, uint32_t) -> uint32_t {
     return atomicAdd(&a.tile_cursor[(size_t)key * kCtrStride], n);
   };
   // The next region's count and first 64 items are loaded while the current
@@ -708,9 +847,9 @@ __global__ __launch_bounds__(256, 8) void k_bin(BatchArgs a) {
         tbase = a.draws[dd].tile_base;
         tiles_x = a.draws[dd].tiles_x;
       }
-      ckpt_alloc(a, act, it, sw);
-      wave_tile_pairs<true>(a, act, it.id, it.draw, tbase, tiles_x, it.bx0 >> sw,
-                            it.bx1 >> sw, it.by0 >> sh, it.by1 >> sh, reserve);
+      ckpt_alloc(a, act, it, sw, (uint32_t)(r * kRegion + j + lane));
+      wave_tile_pairs<true>(a, act, it.id, it.draw, tbase, tiles_x, it.bx0, it.bx1, it.by0,
+                            it.by1, sw, sh, reserve);
     }
     r = rn;
     cnt = cnt_n;
@@ -723,6 +862,53 @@ __global__ __launch_bounds__(256, 8) void k_bin(BatchArgs a) {
   __syncthreads();
   if (threadIdx.x == 0 && blockIdx.x < kStatSlots)
     a.block_stats[kStatSlots + blockIdx.x] = sh_items;
+}
+
+// Row pre-pass: one lane per row task of an item with edge checkpoints (a
+// triangle at least kCkptMinSpan pixels wide spanning several tile
+// columns): walk the row once from the bbox's left column (Renderer.cpp:
+// 240-243) and store the two edge values where it enters each later tile
+// column.  The raster starts such rows from these checkpoints and shading
+// from the one of its pixel's tile column: the same serial chain of
+// additions as one walk from the left column, done once per row instead of
+// once per (row, tile column).  Tasks sit in kCkptSegs segments; a
+// wavefront takes 64 consecutive tasks of one segment.
+__global__ __launch_bounds__(256) void k_rows(BatchArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int sw = __ffs(a.tile_w) - 1;
+  const u64 seg_rows = (a.ckpt_cap / kCkptSegs) / 2;
+  // Tasks per segment (lane s holds segment s) and the largest count.
+  const u64 mine = min(a.ckpt_ctr[lane * kCtrStride] >> 40, seg_rows);
+  u64 most = mine;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) most = max(most, (u64)__shfl_xor(most, o));
+  const u64 nw = (u64)gridDim.x * 4;
+  for (u64 u = (u64)blockIdx.x * 4 + (threadIdx.x >> 6); (u / kCkptSegs) * 64 < most; u += nw) {
+    const int seg = (int)(u % kCkptSegs);
+    const u64 first = (u / kCkptSegs) * 64;
+    const u64 count = __shfl(mine, seg);
+    if (first >= count) continue;
+    const u64 t = first + lane;
+    const u64 task = t < count ? a.ckpt_rows[(u64)seg * seg_rows + t] : ~0ull;
+    if (task == ~0ull) continue;
+    const BinItem it = a.items[task >> 32];
+    const int row = (int)(uint32_t)task;
+    const int64_t g = it.id >> 1;
+    const DrawDesc& D = a.draws[it.draw];
+    TriSetup s;
+    tri_setup_sub(a, D, a.jobs[D.job], (int)(g - D.tri_begin), (int)(it.id & 1u), s);
+    const int2 cb = a.ckpt_base[it.id];
+    float e1, e2;
+    edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y, (float)s.bx0 + 0.5f,
+               (float)row + 0.5f, e1, e2);
+    int c = s.bx0;
+    float2* out = a.ckpt + ((int64_t)cb.x + (int64_t)row * cb.y);
+    for (int tx = (s.bx0 >> sw) + 1; tx <= (s.bx1 >> sw); tx++) {
+      advance2(e1, s.n1x, e2, s.n2x, (tx << sw) - c);
+      c = tx << sw;
+      out[tx] = make_float2(e1, e2);
+    }
+  }
 }
 
 // Depth test of one pixel into the tile's LDS key buffer.
@@ -759,11 +945,19 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
   const int64_t b = blockIdx.x;
   int tile, part;
   uint32_t cnt, off;  // the tile's bin entries: number and first
+  int band_lo = 0, band_hi = TH - 1;  // tile rows of this work item
   if (b < nparts_all) {
     const uint2 pp = a.parts[b];
     tile = (int)pp.x;
     part = (int)pp.y;
-    cnt = a.tile_count[(size_t)tile * kCtrStride];
+    if (pp.y & kBandPart) {  // one row band of the tile, all of its entries
+      const int nb = (int)((pp.y >> 8) & 0xFFu), bi = (int)(pp.y & 0xFFu);
+      const int bh = (TH + nb - 1) / nb;
+      band_lo = bi * bh;
+      band_hi = min(TH - 1, band_lo + bh - 1);
+      part = 0;
+    }
+    cnt = (uint32_t)a.tile_count[(size_t)tile * kCtrStride64];
     off = a.tile_off[tile];
   } else {
     // Tiles rastered whole come from the scan's list, with their bin range
@@ -810,7 +1004,7 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
           const DrawDesc& D = a.draws[be.y];
           tri_setup_sub(a, D, J, (int)(g - D.tri_begin), (int)(id & 1u), s);
         }
-        nrows = min(s.by1, y0 + TH - 1) - max(s.by0, y0) + 1;
+        nrows = max(min(s.by1, y0 + band_hi) - max(s.by0, y0 + band_lo) + 1, 0);
         setups[tid] = s;
         setup_id[tid] = id;
       }
@@ -837,19 +1031,29 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
       for (int q = tid; q < total; q += NT) {
         const int i = owner_search<NT>(row_start, q);
         const TriSetup& s = setups[i];
-        const int row = max(s.by0, y0) + (q - row_start[i]);
+        const int row = max(s.by0, y0 + band_lo) + (q - row_start[i]);
         const int cx0 = max(s.bx0, x0), cx1 = min(s.bx1, x0 + TW - 1);
         const uint32_t id = setup_id[i];
-        // A row entering a tile column after its triangle's first leaves a
-        // checkpoint there; the base is loaded first, so its latency overlaps
-        // the walk.
-        int2 cb = make_int2(0, 0);
-        if (cx0 > s.bx0) cb = a.ckpt_base[id];
-        float e1, e2;
-        edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
-                   (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
-        advance2(e1, s.n1x, e2, s.n2x, cx0 - s.bx0);
-        if (cb.y) a.ckpt[(int64_t)cb.x + (int64_t)row * cb.y + x0 / TW] = make_float2(e1, e2);
+        // A row entering a tile column after its triangle's first starts
+        // from the edge checkpoint the row pre-pass left there (the same
+        // serial chain from the bbox's left column); without checkpoints
+        // (none allocated) it walks from the left column itself.
+        float e1 = 0.0f, e2 = 0.0f;
+        bool from_ckpt = false;
+        if (cx0 > s.bx0 && s.bx1 - s.bx0 + 1 >= kCkptMinSpan) {  // only these have any
+          const int2 cb = a.ckpt_base[id];
+          if (cb.y) {
+            const float2 st = a.ckpt[(int64_t)cb.x + (int64_t)row * cb.y + x0 / TW];
+            e1 = st.x;
+            e2 = st.y;
+            from_ckpt = true;
+          }
+        }
+        if (!from_ckpt) {
+          edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
+                     (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
+          advance2(e1, s.n1x, e2, s.n2x, cx0 - s.bx0);
+        }
         const int prow = (row - y0) * KS - x0;
         for (int c = cx0; c <= cx1; c++) {
           raster_pixel(keys, prow + c, e1, e2, s, persp, id);
@@ -860,7 +1064,8 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
       __syncthreads();
     }
     if (nparts == 1) {
-      for (int p = tid; p < NP; p += NT) {
+      // (a band writes its own rows only)
+      for (int p = tid + band_lo * TW; p < (band_hi + 1) * TW; p += NT) {
         const int c = x0 + (p % TW), row = y0 + (p / TW), lp = (p / TW) * KS + (p % TW);
         if (c < J.W && row < J.H) {
           const size_t pix = (size_t)row * J.W + c;
@@ -1066,33 +1271,48 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   if (J.points) store_point(J, pix, c, row, z);
 }
 
+// A pixel nothing was drawn on: clear(), Renderer.cpp:113-119.
+__device__ inline void shade_background(const JobDesc& J, size_t pix, int c, int row) {
+  if (J.rgb8) {
+    J.rgb8[pix * 3 + 0] = quantize(J.bg[0]);
+    J.rgb8[pix * 3 + 1] = quantize(J.bg[1]);
+    J.rgb8[pix * 3 + 2] = quantize(J.bg[2]);
+  }
+  if (J.depth) J.depth[pix] = kClearDepth;
+  if (J.winner) J.winner[pix] = kNoWinner;
+  if (J.rgb) {
+    J.rgb[pix * 3 + 0] = J.bg[0];
+    J.rgb[pix * 3 + 1] = J.bg[1];
+    J.rgb[pix * 3 + 2] = J.bg[2];
+  }
+  if (J.normals && J.save_normals) {
+    J.normals[pix * 3 + 0] = 0.0f;
+    J.normals[pix * 3 + 1] = 0.0f;
+    J.normals[pix * 3 + 2] = 1.0f;
+  }
+  if (J.points) store_point(J, pix, c, row, kClearDepth);
+}
+
 // Deferred shading of one pixel from its winner (Renderer.cpp:271-305).
 __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
                             const int64_t* dcache, int ncached, uint32_t id,
                             int c, int row) {
   const size_t pix = (size_t)row * (size_t)J.W + (size_t)c;
-  if (id == kNoWinner) {  // clear(), Renderer.cpp:113-119
-    if (J.rgb8) {
-      J.rgb8[pix * 3 + 0] = quantize(J.bg[0]);
-      J.rgb8[pix * 3 + 1] = quantize(J.bg[1]);
-      J.rgb8[pix * 3 + 2] = quantize(J.bg[2]);
-    }
-    if (J.depth) J.depth[pix] = kClearDepth;
-    if (J.winner) J.winner[pix] = kNoWinner;
-    if (J.rgb) {
-      J.rgb[pix * 3 + 0] = J.bg[0];
-      J.rgb[pix * 3 + 1] = J.bg[1];
-      J.rgb[pix * 3 + 2] = J.bg[2];
-    }
-    if (J.normals && J.save_normals) {
-      J.normals[pix * 3 + 0] = 0.0f;
-      J.normals[pix * 3 + 1] = 0.0f;
-      J.normals[pix * 3 + 2] = 1.0f;
-    }
-    if (J.points) store_point(J, pix, c, row, kClearDepth);
+  if (id == kNoWinner) {
+    shade_background(J, pix, c, row);
     return;
   }
   const int d = lookup_draw(dcache, ncached, a, J, (int64_t)(id >> 1));
+  {
+    // A winner id names a triangle of one of the job's draws; anything else
+    // (a winner buffer this batch did not write) is drawn as background.
+    const DrawDesc& Dc = a.draws[d];
+    const int64_t tri = (int64_t)(id >> 1) - Dc.tri_begin;
+    if (tri < 0 || tri >= (int64_t)Dc.ntri) {
+      shade_background(J, pix, c, row);
+      return;
+    }
+  }
   const int d0 = __builtin_amdgcn_readfirstlane(d);
   if (__ballot(d == d0) == __ballot(1)) {
     // One draw for the whole wavefront: its descriptor through scalar loads.
@@ -1169,14 +1389,17 @@ __global__ __launch_bounds__(kShadeBlock, 7) void k_shade(BatchArgs a) {
   for (int64_t i = (int64_t)blockIdx.x * kShadeBlock + tid; i < a.ntiles || i < a.scan_blocks;
        i += (int64_t)a.shade_blocks * kShadeBlock) {
     if (i < a.ntiles) {
-      a.tile_count[(size_t)i * kCtrStride] = 0;
+      a.tile_count[(size_t)i * kCtrStride64] = 0;
       a.tile_done[i] = 0;
     }
     if (i < a.scan_blocks) a.scan_status[i] = 0;
   }
   if (!one_draw) __syncthreads();
   if (blockIdx.x == 0) reduce_stats(a, tid);
-  if (inside) shade_pixel(a, J, dcache, ncached, id, c, row);
+  // After a bin-list or part-list overflow some tiles were not rastered (their
+  // winner ids are stale): nothing is shaded, the batch is re-run.
+  const bool overflow = a.counters[kBinOverflow] != 0ull || a.counters[kHeavyOverflow] != 0ull;
+  if (inside && !overflow) shade_pixel(a, J, dcache, ncached, id, c, row);
 }
 
 // ---- scratch of the entry point that allocates nothing ----------------------
@@ -1260,6 +1483,11 @@ hipError_t launch_scan(const BatchArgs& a, hipStream_t s) {
 
 hipError_t launch_bin(const BatchArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_bin, dim3(bin_grid(a)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rows(const BatchArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_rows, dim3(bin_grid(a)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -12,8 +12,11 @@ struct BatchArgs {
   BinItem* items;            // [nregions * kRegion] survivors, per wavefront
   uint32_t* region_count;    // [nregions] survivors of each wavefront
   int64_t nregions;
-  uint32_t* tile_count;      // [ntiles] (tile, triangle) pairs per tile
-  uint32_t* tile_off;        // [ntiles] exclusive scan of tile_count
+  // [ntiles * kCtrStride64] per tile: (walk work << 32) | (tile, triangle)
+  // pairs, where a pair's walk work is its bbox pixels in the tile + 4 per
+  // row + 16 (for the group leader's pair, times the group size)
+  unsigned long long* tile_count;
+  uint32_t* tile_off;        // [ntiles] exclusive scan of the pair counts
   uint32_t* tile_cursor;     // [ntiles] scatter cursors
   uint32_t* tile_heavy;      // [ntiles] merge slot of a split tile, kNoSlot, or kEmptyTile
   uint4* tile_list;          // [ntiles] {tile, first bin entry, entries, 0} of the tiles
@@ -33,9 +36,15 @@ struct BatchArgs {
   float2* ckpt;              // [ckpt_cap], kCkptSegs equal segments
   uint64_t ckpt_cap;
   unsigned long long* ckpt_ctr;  // [kCkptSegs * kCtrStride] segment fill counters
+  // Row tasks of the row pre-pass, (item index << 32) | pixel row, in
+  // kCkptSegs segments of (ckpt_cap / kCkptSegs) / 2 (a task per row of an
+  // item with checkpoints, each row holding two entries or more); the
+  // segment counters ckpt_ctr count them in their high 24 bits.
+  unsigned long long* ckpt_rows;
   unsigned long long* heavy_keys;  // [heavy_cap * tile pixels] merge buffers
   uint32_t heavy_cap;
   uint32_t frag_size;        // max (tile, triangle) pairs per raster work item
+  uint32_t band_work;        // a tile with more walk work is rastered in row bands
   unsigned long long* scan_status;  // [scan_blocks] decoupled look-back words
   unsigned long long* counters;     // [kNumCounters]
   // Per-workgroup statistics (no shared counter: same-address atomics
@@ -101,6 +110,7 @@ hipError_t launch_vertex(const BatchArgs& a, hipStream_t s);
 hipError_t launch_geometry(const BatchArgs& a, hipStream_t s);
 hipError_t launch_scan(const BatchArgs& a, hipStream_t s);
 hipError_t launch_bin(const BatchArgs& a, hipStream_t s);
+hipError_t launch_rows(const BatchArgs& a, hipStream_t s);
 hipError_t launch_tile(const BatchArgs& a, hipStream_t s);
 // Geometry workgroups one CU holds at once (from the kernel's resources).
 int geom_blocks_per_cu();

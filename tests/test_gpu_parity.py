@@ -302,3 +302,27 @@ def test_no_near_clip_draws(gpu_ctx):
                                scenes.translate(0.0, 0.0, -1.5))],
                    light=(0.5, 1.0, 2.0), light_is_point=True)
     check(gpu_ctx, fr, "unclipped sphere around the eye")
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_row_bands(gpu_ctx, tile):
+    """Tiles with much walk work are rastered as row bands, one workgroup per
+    band with all of the tile's entries: a tiny threshold makes nearly every
+    tile a banded one (large and clipped triangles, checkpoints, textures,
+    crowded tiles split by entries next to banded ones)."""
+    gpu_ctx.set_tile(tile, tile)
+    gpu_ctx.set_band_work(64)
+    try:
+        for seed in (110, 111):
+            check(gpu_ctx, fuzz_frame(seed, 160, 100), f"bands t{tile} seed {seed}")
+        check(gpu_ctx, scenes.config_terrain(n=96, width=320, height=180, shapes=True),
+              f"bands terrain t{tile}")
+        check(gpu_ctx, stack_frame(1500, 80, 56, seed=tile), f"bands split t{tile}")
+        rng_tris = scenes.random_triangles(40, 21, lo=0.2, hi=1.5, center_lo=(-1, -1, -3),
+                                           center_hi=(1, 1, -2))
+        fr = FrameSpec(640, 360, scenes.projection_frustum(math.radians(60), 16 / 9, 0.1, 50.0),
+                       draws=[Draw(rng_tris, Material(shininess=40.0))], light=(0.3, 0.5, 1.0))
+        check(gpu_ctx, fr, f"bands large triangles t{tile}")
+    finally:
+        gpu_ctx.set_band_work(32768)
+        gpu_ctx.set_tile(16, 16)
