@@ -738,6 +738,10 @@ __device__ inline void ckpt_alloc(const BatchArgs& a, bool act, const BinItem& i
   const int ntx = act ? (it.bx1 >> sw) - btx + 1 : 0;
   const int rows = it.by1 - it.by0 + 1;
   const bool want = ntx > 1 && it.bx1 - it.bx0 + 1 >= kCkptMinSpan;
+  if (__ballot(want) == 0ull) {  // the common case: no wide triangle here
+    if (act) a.ckpt_base[it.id] = make_int2(0, 0);
+    return;
+  }
   const u64 need = want ? (u64)ntx * (u64)rows : 0ull;
   const uint32_t nrow = want ? (uint32_t)rows : 0u;
   u64 P = need;  // inclusive prefixes over the wave
