@@ -84,6 +84,8 @@ def parse():
     p.add_argument("--p99-budget", type=float, default=50.0, help="ms")
     p.add_argument("--serve-batch-profile", action="store_true",
                    help="only time one batch of --serve-batch serving requests (GPU stages)")
+    p.add_argument("--persistent", action="store_true",
+                   help="with a batch profile: one persistent launch per batch (16 x 16 tiles)")
     p.add_argument("--target-batch-profile", action="store_true",
                    help="only time one batch of --serve-batch 1M-triangle 1080p requests")
     return p.parse_args()
@@ -401,11 +403,34 @@ def batched_section(args, torch, mra):
         plain()
     torch.cuda.synchronize()
     tp = (time.perf_counter() - t0) / max(1, reps // 2)
+    # The same pass at 16 x 16 tiles as separate launches and as one launch
+    # of the persistent kernel (which runs 16 x 16 tiles only).
+    ways = {}
+    ctx.set_tile(16, 16)
+    for name, on in (("launches_16", False), ("persistent_16", True)):
+        ctx.set_persistent(on)
+        batch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            batch()
+        torch.cuda.synchronize()
+        tw = (time.perf_counter() - t0) / reps
+        ctx.set_profiling(True)
+        gms = []
+        for _ in range(3):
+            batch()
+            gms.append(ctx.stats().frame_ms)
+        ctx.set_profiling(False)
+        ways[name] = {"ms_per_pass": round(tw * 1e3, 3), "gpu_ms": round(float(np.median(gms)), 3),
+                      "launches": int(ctx.stats().launches)}
+        log(f"batched {name}: {ways[name]}")
+    ctx.set_persistent(False)
     ctx.close()
     return {"jobs": n, "triangles": tris, "autotune": tune,
             "batched": {"ms_per_pass": tb * 1e3, "frames_per_s": n / tb, "Mtris_per_s": tris / tb / 1e6},
             "plain": {"ms_per_pass": tp * 1e3, "frames_per_s": n / tp, "Mtris_per_s": tris / tp / 1e6},
-            "speedup": tp / tb}
+            "speedup": tp / tb, "tile16": ways}
 
 
 def serve_batch_profile(args, torch, mra, target=False):
@@ -447,6 +472,10 @@ def serve_batch_profile(args, torch, mra, target=False):
                 "frag_size": args.frag_size, "fixed": True}
     else:
         tune = ctx.autotune(reqs, outs=outs, reps=2, cache_path=args.tune_cache)
+    if args.persistent:
+        ctx.set_tile(16, 16)
+        ctx.set_persistent(True)
+        tune = dict(tune, tile=16, raster_threads=256, persistent=True)
 
     def batch():
         rc = ctx.lib.mr_render_batch(ctx.h, farr, n, oarr)
@@ -469,8 +498,9 @@ def serve_batch_profile(args, torch, mra, target=False):
         st.append(list(s.stage_ms[:len(STAGES)]))
         fm.append(s.frame_ms)
     med = np.median(np.array(st), 0)
+    launches = int(ctx.stats().launches)
     ctx.close()
-    return {"serve_batch": n, "triangles": sum(r.ntri for r in reqs),
+    return {"serve_batch": n, "launches": launches, "triangles": sum(r.ntri for r in reqs),
             "pixels": sum(r.width * r.height for r in reqs), "autotune": tune,
             "ms_per_batch": round(ms, 4), "requests_per_s": round(n / ms * 1e3, 1),
             "gpu_ms": round(float(np.median(fm)), 4),

@@ -101,8 +101,12 @@ typedef struct mr_stats {
   int64_t retries;            /* re-runs after a bin-capacity overflow           */
   double stage_ms[8];         /* per-stage GPU time when profiling is on:        */
                               /* 0 clear, 1 geometry, 2 scan, 3 bin, 4 rows      */
-                              /* (edge checkpoints), 5 raster, 6 shade            */
+                              /* (edge checkpoints), 5 raster, 6 shade; for the   */
+                              /* persistent kernel each stage's span (its first   */
+                              /* workgroup's start to its last one's end)         */
   double frame_ms;            /* first to last kernel of the frame (profiling)  */
+  int64_t launches;           /* kernel launches of the last batch (1: the     */
+                              /* persistent kernel)                             */
 } mr_stats;
 
 /* ---- device context: HIP device, stream, resident meshes/textures ------- */
@@ -171,6 +175,23 @@ int mr_ctx_set_bin_factor(mr_ctx* ctx, float entries_per_triangle);
  * thread is the bottleneck.  No counterpart in the reference (host-side
  * launch overhead). */
 int mr_ctx_set_graphs(mr_ctx* ctx, int32_t on);
+
+/* Run each batch of the synchronous calls (mr_render, mr_render_batch) as
+ * ONE launch of a persistent kernel: a resident grid whose workgroups take
+ * the stages' work from device work queues (one per XCD), each stage
+ * starting once the stages before it are complete (bounded waits: a wait
+ * over 2 s ends the batch with MR_ERR_DEVICE).  Same stages, bit-identical
+ * results to the separate launches.  16 x 16 tiles only (MR_ERR_ARG
+ * otherwise; a later tile change makes batches run as separate launches
+ * again, see mr_stats.launches); the service and mr_render_noalloc always
+ * use separate launches.  Off by default: measured on MI355X it is slower
+ * at every size (1M-triangle 1080p frame 1.33 ms vs 0.148 ms; 64 such frames
+ * 9.5 ms vs 4.6 ms), because each stage hand-over makes every workgroup
+ * release and acquire at agent scope (an L2 write-back and invalidate per
+ * workgroup on a GPU of eight L2s), where a kernel boundary does one of each.
+ * Replaces the reference's per-frame call loop (samples/bench.cpp:128-135,
+ * src/Renderer.cpp:311-332). */
+int mr_ctx_set_persistent(mr_ctx* ctx, int32_t on);
 /* Record per-stage HIP event times into mr_stats (adds event overhead). */
 int mr_ctx_set_profiling(mr_ctx* ctx, int32_t on);
 int mr_ctx_get_stats(mr_ctx* ctx, mr_stats* stats);

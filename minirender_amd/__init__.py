@@ -151,6 +151,10 @@ class Context:
     def set_graphs(self, on: bool) -> None:
         check(self.lib.mr_ctx_set_graphs(self.h, int(on)), "mr_ctx_set_graphs")
 
+    def set_persistent(self, on: bool) -> None:
+        """Synchronous batches as one launch of the persistent kernel."""
+        check(self.lib.mr_ctx_set_persistent(self.h, int(on)), "mr_ctx_set_persistent")
+
     def set_stream(self, stream: int) -> None:
         check(self.lib.mr_ctx_set_stream(self.h, C.c_void_p(stream)), "mr_ctx_set_stream")
 

@@ -63,6 +63,7 @@ class mr_stats(C.Structure):
         ("triangles_in", C.c_int64), ("triangles_clipped", C.c_int64),
         ("records", C.c_int64), ("bin_entries", C.c_int64), ("tiles", C.c_int64),
         ("retries", C.c_int64), ("stage_ms", C.c_double * 8), ("frame_ms", C.c_double),
+        ("launches", C.c_int64),
     ]
 
 
@@ -108,6 +109,7 @@ SIGNATURES = {
     "mr_ctx_set_frag_size": (C.c_int, [_P, _I]),
     "mr_ctx_set_profiling": (C.c_int, [_P, _I]),
     "mr_ctx_set_graphs": (C.c_int, [_P, _I]),
+    "mr_ctx_set_persistent": (C.c_int, [_P, _I]),
     "mr_ctx_get_stats": (C.c_int, [_P, C.POINTER(mr_stats)]),
     "mr_ctx_memory": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "mr_mesh_upload": (C.c_int, [_P, _P, _I, _P, _I, _P, _I, _P, _P, _P, _I, _IP]),

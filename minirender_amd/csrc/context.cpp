@@ -111,7 +111,7 @@ void slot_free(mr_ctx* c, Slot& s) {
   DevBuf* bufs[] = {&x.items, &x.region_count, &x.tile_count, &x.tile_off,
                     &x.tile_cursor, &x.tile_heavy, &x.tile_list, &x.tile_done, &x.parts, &x.bins,
                     &x.ckpt_base, &x.ckpt, &x.ckpt_ctr, &x.heavy_keys, &x.scan_status, &x.counters, &x.block_stats,
-                    &x.tables, &x.vrec, &x.win, &x.wide, &x.outputs};
+                    &x.tables, &x.vrec, &x.win, &x.wide, &x.outputs, &x.ctl};
   for (DevBuf* b : bufs) dev_free(c, *b, true);
   HostBuf* hbufs[] = {&x.h_tables, &x.h_counters, &x.h_outputs};
   for (HostBuf* b : hbufs) host_free(*b);
@@ -337,6 +337,10 @@ static int ensure_scratch(mr_ctx* c, Slot& sl, const BatchPlan& p, uint64_t bins
     if ((rc = dev_alloc(c, *bufs[k], bytes[k], true))) return rc;
   if ((rc = host_alloc(s.h_tables, p.layout.bytes))) return rc;
   if ((rc = host_alloc(s.h_counters, kNumCounters * 8))) return rc;
+  if (c->persistent && &sl == &c->main && !s.ctl.p) {
+    if ((rc = dev_alloc(c, s.ctl, kCtlBytes + kStampBytes, true))) return rc;
+    HIP_TRY(hipMemsetAsync(s.ctl.p, 0, kCtlBytes + kStampBytes, sl.stream));
+  }
   return MR_OK;
 }
 
@@ -410,8 +414,23 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   return a;
 }
 
-static int launch_stages(const BatchArgs& a, Slot& s, bool profile, bool clear) {
+// persistent: every stage in one launch of the persistent kernel (profiling
+// then times the whole launch as stage 0).
+static int launch_stages(const BatchArgs& a, Slot& s, bool profile, bool clear,
+                         bool persistent) {
   hipStream_t st = s.stream;
+  if (persistent) {
+    unsigned long long* ctl = (unsigned long long*)s.sc.ctl.p;
+    unsigned long long* stamps = profile ? ctl + kCtlBytes / 8 : nullptr;
+    if (profile) HIP_TRY(hipMemsetAsync(stamps, 0, kStampBytes, st));
+    if (profile) HIP_TRY(hipEventRecord(s.ev[0], st));
+    HIP_TRY(launch_persistent(a, clear, ctl, stamps, st));
+    if (profile)
+      for (int i = 1; i <= kStages; i++) HIP_TRY(hipEventRecord(s.ev[i], st));
+    s.last_launches = 1;
+    return MR_OK;
+  }
+  s.last_launches = (clear ? 1 : 0) + (a.total_verts ? 1 : 0) + (a.total_tris ? 1 : 0) + 5;
   if (profile) HIP_TRY(hipEventRecord(s.ev[0], st));
   if (clear) HIP_TRY(launch_clear(a, st));
   if (profile) HIP_TRY(hipEventRecord(s.ev[1], st));
@@ -477,10 +496,14 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   const int64_t next_tiles = std::max<int64_t>(p.ntiles, same_bufs ? s.clean_tiles : 0);
   const int64_t next_scan = std::max<int64_t>(a.scan_blocks, same_bufs ? s.clean_scan : 0);
   s.clean_tiles = s.clean_scan = 0;  // until the launches below succeed
+  // The persistent kernel runs for the synchronous calls only: two of them
+  // in flight at once (the service's slots) would compete for residency.
+  const bool persistent = c->persistent && &s == &c->main && x.ctl.p &&
+                          persistent_supported(c->tile_w, c->tile_h);
   auto issue = [&]() -> int {
     HIP_TRY(hipMemcpyAsync(x.tables.p, x.h_tables.p, p.layout.bytes, hipMemcpyHostToDevice,
                            s.stream));
-    int r = launch_stages(a, s, profile, clear);
+    int r = launch_stages(a, s, profile, clear, persistent);
     if (r) return r;
     HIP_TRY(hipMemcpyAsync(x.h_counters.p, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost,
                            s.stream));
@@ -489,8 +512,8 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   // Graphs only on the slot's own stream (a caller's stream may be the null
   // stream, which cannot be captured) and without per-stage events.
   if (c->graphs && !profile && s.own_stream && s.stream == s.own_stream) {
-    const void* hp[4] = {x.h_tables.p, x.h_counters.p, (const void*)p.layout.bytes,
-                         (const void*)(uintptr_t)clear};
+    const void* hp[5] = {x.h_tables.p, x.h_counters.p, (const void*)p.layout.bytes,
+                         (const void*)(uintptr_t)clear, persistent ? x.ctl.p : nullptr};
     std::vector<unsigned char> key(sizeof(BatchArgs) + sizeof(hp));
     memcpy(key.data(), &a, sizeof(BatchArgs));
     memcpy(key.data() + sizeof(BatchArgs), hp, sizeof(hp));
@@ -544,7 +567,12 @@ int finish_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   s.retries = 0;
   for (int attempt = 0;; attempt++) {
     HIP_TRY(hipEventSynchronize(s.done));
-    if (hc[kError]) return MR_ERR_DEVICE;
+    if (hc[kError]) {
+      // A bounded wait gave up.  The persistent kernel's last workgroup
+      // clears its control words in any case; cleared again to be sure.
+      if (x.ctl.p) HIP_TRY(hipMemset(x.ctl.p, 0, kCtlBytes));
+      return MR_ERR_DEVICE;
+    }
     if (!hc[kBinOverflow] && !hc[kHeavyOverflow]) break;
     if (attempt >= 3 || hc[kBinEntries] >= 0xFFFFFFF0ull) return MR_ERR_NOMEM;
     s.retries++;
@@ -571,7 +599,29 @@ int finish_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
     st.bin_entries = (int64_t)hc[kBinEntries];
     st.tiles = p.ntiles;
     st.retries = s.retries;
-    if (profile) {
+    st.launches = s.last_launches;
+    if (profile && s.last_launches == 1 && x.ctl.p) {
+      // One persistent launch: each stage's span from its workgroups' stamps
+      // (first start to last end; the stages of the vertex and geometry
+      // kernels are reported together, as the separate launches are).
+      unsigned long long w[2 * kPStages * kStampStride];
+      HIP_TRY(hipMemcpy(w, (const char*)x.ctl.p + kCtlBytes, kStampBytes, hipMemcpyDeviceToHost));
+      const double hz = (double)wall_clock_hz();
+      const int slot[kPStages] = {0, 1, 1, 2, 3, 4, 5, 6};
+      unsigned long long b[kStages], e[kStages];
+      for (int i = 0; i < kStages; i++) b[i] = ~0ull, e[i] = 0;
+      for (int k = 0; k < kPStages; k++) {
+        const unsigned long long nb = w[(2 * k) * kStampStride], ne = w[(2 * k + 1) * kStampStride];
+        if (!nb || !ne) continue;  // the stage had no items
+        b[slot[k]] = std::min(b[slot[k]], ~nb);
+        e[slot[k]] = std::max(e[slot[k]], ne);
+      }
+      for (int i = 0; i < kStages; i++)
+        st.stage_ms[i] = e[i] > b[i] ? (double)(e[i] - b[i]) / hz * 1e3 : 0.0;
+      float ms = 0;
+      hipEventElapsedTime(&ms, s.ev[0], s.ev[kStages]);
+      st.frame_ms = ms;
+    } else if (profile) {
       float ms = 0;
       for (int i = 0; i < kStages; i++) {
         hipEventElapsedTime(&ms, s.ev[i], s.ev[i + 1]);
@@ -699,7 +749,7 @@ static int render_noalloc(mr_ctx* c, const mr_frame* frame, const mr_outputs* ou
   const BatchArgs a = make_args(c, ns, p);
   unsigned long long* hc = g_na_counters[c->device];
   HIP_TRY(hipMemcpyAsync(ns.sc.tables.p, ht, p.layout.bytes, hipMemcpyHostToDevice, ns.stream));
-  if ((rc = launch_stages(a, ns, false, true))) return rc;
+  if ((rc = launch_stages(a, ns, false, true, false))) return rc;
   HIP_TRY(hipMemcpyAsync(hc, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost, ns.stream));
   HIP_TRY(hipStreamSynchronize(ns.stream));
   if (hc[kError]) return MR_ERR_DEVICE;
@@ -846,6 +896,14 @@ int mr_ctx_set_bin_factor(mr_ctx* c, float f) {
 int mr_ctx_set_profiling(mr_ctx* c, int32_t on) {
   if (!c) return MR_ERR_ARG;
   c->profiling = on != 0;
+  return MR_OK;
+}
+
+int mr_ctx_set_persistent(mr_ctx* c, int32_t on) {
+  if (!c) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (on && !persistent_supported(c->tile_w, c->tile_h)) return MR_ERR_ARG;
+  c->persistent = on != 0;
   return MR_OK;
 }
 

@@ -55,6 +55,9 @@ struct Scratch {
   DevBuf items, region_count, tile_count, tile_off, tile_cursor, tile_heavy, tile_list,
       tile_done, parts, bins, ckpt_base, ckpt, ckpt_ctr, heavy_keys, scan_status, counters,
       block_stats, tables, vrec, win, wide, outputs;
+  // Control words of the persistent kernel (kCtlBytes, zero between
+  // launches), then its stage stamps when profiling (kStampBytes).
+  DevBuf ctl;
   // The batch tables (jobs, job tile/shade bases, draws, draw starts) sit in
   // one pinned staging buffer and one device buffer, uploaded by one copy.
   HostBuf h_tables, h_counters, h_outputs;
@@ -97,6 +100,7 @@ struct Slot {
   uint32_t heavy_slots = 0;  // merge buffers of split tiles (grown on demand)
   size_t parts_want = 0;     // raster work items a batch needed (after an overflow)
   int32_t last_geom_grid = 0, last_bin_grid = 0;
+  int32_t last_launches = 0;  // kernel launches of the last batch
   int64_t retries = 0;
   // Steady state: batches' uploads + stages + counter read-back captured as
   // instantiated graphs, keyed by the launch parameters, most recent first
@@ -130,6 +134,7 @@ struct mr_ctx {
   uint32_t band_work = 32768;  // walk work above which a tile is rastered in row bands
   bool profiling = false;
   bool graphs = false;  // capture steady-state batches as HIP graphs
+  bool persistent = false;  // run batches as one persistent launch (16 x 16 tiles)
   mr_stats stats{};
   int32_t num_cus = 256;
   std::recursive_mutex mu;

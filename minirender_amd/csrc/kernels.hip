@@ -86,10 +86,12 @@ __device__ u64 lookback(u64* status, int idx, u64 aggregate, u64* counters) {
   return excl;
 }
 
-__global__ void k_clear(BatchArgs a) {
+// Each stage is a device function of its workgroup index vb in a grid of
+// nvb workgroups of 256 threads: run by its own kernel (vb = blockIdx.x),
+// or by the persistent kernel, whose workgroups take stage work from queues.
+__device__ __forceinline__ void stage_clear(const BatchArgs& a, int vb, int nvb) {
   const int n = max(max(a.scan_blocks, a.ntiles), (int)kNumCounters);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += gridDim.x * blockDim.x) {
+  for (int i = vb * 256 + (int)threadIdx.x; i < n; i += nvb * 256) {
     if (i < kNumCounters) a.counters[i] = 0;
     if (i < a.scan_blocks) a.scan_status[i] = 0;
     if (i < a.ntiles) {
@@ -97,6 +99,10 @@ __global__ void k_clear(BatchArgs a) {
       a.tile_done[i] = 0;
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_clear(BatchArgs a) {
+  stage_clear(a, blockIdx.x, gridDim.x);
 }
 
 // Last index in [lo, hi] with begin[index] <= x (begin ascending).
@@ -321,12 +327,12 @@ __device__ inline void vertex_record(const BatchArgs& a, const DrawDesc& D, cons
   r[2] = make_float2(v.py, v.zv);
 }
 
-__global__ __launch_bounds__(256) void k_vertex(BatchArgs a) {
+__device__ __forceinline__ void stage_vertex(const BatchArgs& a, int vb, int nvb) {
   // Contiguous range per workgroup (a multiple of kVertexStep): one group
   // search, then stepping.
-  const int64_t chunk = ((a.total_verts + (int64_t)gridDim.x * kVertexStep - 1) /
-                         ((int64_t)gridDim.x * kVertexStep)) * kVertexStep;
-  const int64_t r_begin = (int64_t)blockIdx.x * chunk;
+  const int64_t chunk = ((a.total_verts + (int64_t)nvb * kVertexStep - 1) /
+                         ((int64_t)nvb * kVertexStep)) * kVertexStep;
+  const int64_t r_begin = (int64_t)vb * chunk;
   const int64_t r_end = min(r_begin + chunk, a.total_verts);
   const int lane = threadIdx.x & 63;
   int gw = search_last(a.vg_begin, 0, a.nvgroups - 1,
@@ -367,6 +373,10 @@ __global__ __launch_bounds__(256) void k_vertex(BatchArgs a) {
       }
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_vertex(BatchArgs a) {
+  stage_vertex(a, blockIdx.x, gridDim.x);
 }
 
 // Pairs of one wavefront step: lanes with ok set hold a pair (tile key, id,
@@ -474,20 +484,20 @@ __device__ void wave_tile_pairs(const BatchArgs& a, bool active, uint32_t id, ui
 
 // (a)+(b): one lane per input triangle; wavefront w writes its survivors to
 // items[w * kRegion ...] and their number to region_count[w].
-__global__ __launch_bounds__(256, 7) void k_geom(BatchArgs a) {
+__device__ __forceinline__ void stage_geom(const BatchArgs& a, int vb, int nvb) {
   // The batch counters and checkpoint segment counters are first used by the
   // scan and the binner: clear them here.
-  if (blockIdx.x == 0 && threadIdx.x < kNumCounters) a.counters[threadIdx.x] = 0;
-  if (blockIdx.x == 0 && threadIdx.x < kCkptSegs) a.ckpt_ctr[threadIdx.x * kCtrStride] = 0;
+  if (vb == 0 && threadIdx.x < kNumCounters) a.counters[threadIdx.x] = 0;
+  if (vb == 0 && threadIdx.x < kCkptSegs) a.ckpt_ctr[threadIdx.x * kCtrStride] = 0;
   const int lane = threadIdx.x & 63;
   const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
   uint32_t nclipped = 0;
   // Each workgroup takes one contiguous range of triangles (a multiple of
   // 256), so a wavefront finds its draw once and then only steps over draw
   // boundaries; its draw and job descriptors are read with scalar loads.
-  const int64_t chunk = ((a.total_tris + (int64_t)gridDim.x * 256 - 1) /
-                         ((int64_t)gridDim.x * 256)) * 256;
-  const int64_t r_begin = (int64_t)blockIdx.x * chunk;
+  const int64_t chunk = ((a.total_tris + (int64_t)nvb * 256 - 1) /
+                         ((int64_t)nvb * 256)) * 256;
+  const int64_t r_begin = (int64_t)vb * chunk;
   const int64_t r_end = min(r_begin + chunk, a.total_tris);
   int dw = search_last(a.draw_begin, 0, a.ndraws - 1,
                        (int64_t)__builtin_amdgcn_readfirstlane((int)min(r_begin, a.total_tris)));
@@ -559,7 +569,11 @@ __global__ __launch_bounds__(256, 7) void k_geom(BatchArgs a) {
   __syncthreads();
   if (lane == 0 && nclipped) atomicAdd(&sh_clipped, nclipped);
   __syncthreads();
-  if (threadIdx.x == 0 && blockIdx.x < kStatSlots) a.block_stats[blockIdx.x] = sh_clipped;
+  if (threadIdx.x == 0 && vb < kStatSlots) a.block_stats[vb] = sh_clipped;
+}
+
+__global__ __launch_bounds__(256, 7) void k_geom(BatchArgs a) {
+  stage_geom(a, blockIdx.x, gridDim.x);
 }
 
 // Row bands of a tile rastered by several workgroups (each with all of the
@@ -582,7 +596,7 @@ __device__ inline uint32_t band_count(const BatchArgs& a, uint32_t cnt, uint32_t
 // cursors; tiles with more than frag_size pairs get a merge slot and their
 // parts are listed for the raster, tiles with much walk work their row
 // bands.
-__global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
+__device__ __forceinline__ void stage_scan(const BatchArgs& a) {
   __shared__ int sh_blk;
   __shared__ u64 sh_w[kScanBlock / 64];
   __shared__ u64 sh_excl;
@@ -716,6 +730,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
   }
 }
 
+__global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) { stage_scan(a); }
+
 // Edge checkpoints of the wavefront's items: an item at least
 // kCkptMinSpan pixels wide that spans ntx > 1 tile columns gets ntx entries
 // per pixel row (one per column, where the row enters it) and one row task
@@ -732,7 +748,7 @@ constexpr int kCkptMinSpan = 64;
 constexpr u64 kCkptEntryMask = (1ull << 40) - 1;
 
 __device__ inline void ckpt_alloc(const BatchArgs& a, bool act, const BinItem& it, int sw,
-                                  uint32_t item) {
+                                  uint32_t item, int vb) {
   const int lane = threadIdx.x & 63;
   const int btx = it.bx0 >> sw;
   const int ntx = act ? (it.bx1 >> sw) - btx + 1 : 0;
@@ -763,7 +779,7 @@ __device__ inline void ckpt_alloc(const BatchArgs& a, bool act, const BinItem& i
   const uint32_t rtotal = __shfl(R, 63);
   const u64 seg_cap = a.ckpt_cap / kCkptSegs;
   const u64 seg_rows = seg_cap / 2;
-  const int seg = (int)((blockIdx.x * 4u + (threadIdx.x >> 6)) % kCkptSegs);
+  const int seg = (int)(((uint32_t)vb * 4u + (threadIdx.x >> 6)) % kCkptSegs);
   u64 old = 0;
   if (lane == 63) old = atomicAdd(&a.ckpt_ctr[seg * kCtrStride], ((u64)rtotal << 40) | total);
   old = __shfl(old, 63);
@@ -801,15 +817,14 @@ __device__ inline void ckpt_alloc(const BatchArgs& a, bool act, const BinItem& i
 
 // (c) scatter triangle ids into the bin lists of the tiles they overlap and
 // take their edge checkpoints; also resets the merge buffers of split tiles.
-__global__ __launch_bounds__(256, 8) void k_bin(BatchArgs a) {
+__device__ __forceinline__ void stage_bin(const BatchArgs& a, int vb, int nvb) {
   const int lane = threadIdx.x & 63;
   const u64 np = (u64)a.tile_w * a.tile_h;
   const u64 nheavy = min(a.counters[kHeavy], (u64)a.heavy_cap);
-  for (u64 i = (u64)blockIdx.x * 256 + threadIdx.x; i < nheavy * np;
-       i += (u64)gridDim.x * 256)
+  for (u64 i = (u64)vb * 256 + threadIdx.x; i < nheavy * np; i += (u64)nvb * 256)
     a.heavy_keys[i] = ~0ull;
   const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
-  const int64_t nw = (int64_t)gridDim.x * 4;
+  const int64_t nw = (int64_t)nvb * 4;
   u64 nitems = 0;
   auto reserve = [&](int key, uint32_t n, uint32_t) -> uint32_t {
     return atomicAdd(&a.tile_cursor[(size_t)key * kCtrStride], n);
@@ -817,7 +832,7 @@ __global__ __launch_bounds__(256, 8) void k_bin(BatchArgs a) {
   // The next region's count and first 64 items are loaded while the current
   // region is scattered (both loads are independent: every region slot is
   // allocated, slots past the count are read but not used).
-  int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  int64_t r = (int64_t)vb * 4 + (threadIdx.x >> 6);
   uint32_t cnt = 0;
   BinItem first = BinItem{0, 0, 0, 0, 0, 0};
   if (r < a.nregions) {
@@ -851,7 +866,7 @@ __global__ __launch_bounds__(256, 8) void k_bin(BatchArgs a) {
         tbase = a.draws[dd].tile_base;
         tiles_x = a.draws[dd].tiles_x;
       }
-      ckpt_alloc(a, act, it, sw, (uint32_t)(r * kRegion + j + lane));
+      ckpt_alloc(a, act, it, sw, (uint32_t)(r * kRegion + j + lane), vb);
       wave_tile_pairs<true>(a, act, it.id, it.draw, tbase, tiles_x, it.bx0, it.bx1, it.by0,
                             it.by1, sw, sh, reserve);
     }
@@ -864,8 +879,11 @@ __global__ __launch_bounds__(256, 8) void k_bin(BatchArgs a) {
   __syncthreads();
   if (lane == 0 && nitems) atomicAdd(&sh_items, nitems);
   __syncthreads();
-  if (threadIdx.x == 0 && blockIdx.x < kStatSlots)
-    a.block_stats[kStatSlots + blockIdx.x] = sh_items;
+  if (threadIdx.x == 0 && vb < kStatSlots) a.block_stats[kStatSlots + vb] = sh_items;
+}
+
+__global__ __launch_bounds__(256, 8) void k_bin(BatchArgs a) {
+  stage_bin(a, blockIdx.x, gridDim.x);
 }
 
 // Row pre-pass: one lane per row task of an item with edge checkpoints (a
@@ -877,7 +895,7 @@ __global__ __launch_bounds__(256, 8) void k_bin(BatchArgs a) {
 // additions as one walk from the left column, done once per row instead of
 // once per (row, tile column).  Tasks sit in kCkptSegs segments; a
 // wavefront takes 64 consecutive tasks of one segment.
-__global__ __launch_bounds__(256) void k_rows(BatchArgs a) {
+__device__ __forceinline__ void stage_rows(const BatchArgs& a, int vb, int nvb) {
   const int lane = threadIdx.x & 63;
   const int sw = __ffs(a.tile_w) - 1;
   const u64 seg_rows = (a.ckpt_cap / kCkptSegs) / 2;
@@ -886,8 +904,8 @@ __global__ __launch_bounds__(256) void k_rows(BatchArgs a) {
   u64 most = mine;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) most = max(most, (u64)__shfl_xor(most, o));
-  const u64 nw = (u64)gridDim.x * 4;
-  for (u64 u = (u64)blockIdx.x * 4 + (threadIdx.x >> 6); (u / kCkptSegs) * 64 < most; u += nw) {
+  const u64 nw = (u64)nvb * 4;
+  for (u64 u = (u64)vb * 4 + (threadIdx.x >> 6); (u / kCkptSegs) * 64 < most; u += nw) {
     const int seg = (int)(u % kCkptSegs);
     const u64 first = (u / kCkptSegs) * 64;
     const u64 count = __shfl(mine, seg);
@@ -915,6 +933,10 @@ __global__ __launch_bounds__(256) void k_rows(BatchArgs a) {
   }
 }
 
+__global__ __launch_bounds__(256) void k_rows(BatchArgs a) {
+  stage_rows(a, blockIdx.x, gridDim.x);
+}
+
 // Depth test of one pixel into the tile's LDS key buffer.
 __device__ inline void raster_pixel(u64* keys, int p, float e1, float e2,
                                     const TriSetup& s, bool persp, uint32_t id) {
@@ -929,8 +951,10 @@ __device__ inline void raster_pixel(u64* keys, int p, float e1, float e2,
 // (d): one workgroup per raster work item: first the parts of split tiles
 // (listed by the scan), then every tile.  The grid has parts_cap + ntiles
 // workgroups; workgroups past the actual number of items exit at once.
+// remap: listed tiles in XCD-interleaved order (own launch, where
+// consecutive workgroups go to different XCDs), else in list order.
 template <int TW, int TH, int NT>
-__global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
+__device__ __forceinline__ void stage_tile(const BatchArgs& a, int64_t b, bool remap) {
   constexpr int NP = TW * TH;
   // Key rows padded by one entry: pixels of one column in different rows
   // fall in different LDS banks.
@@ -946,7 +970,6 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
   // split tiles are then skipped (the frame is re-run with larger buffers).
   const int64_t nparts_all =
       a.counters[kHeavyOverflow] ? 0 : (int64_t)min(a.counters[kParts], (u64)a.parts_cap);
-  const int64_t b = blockIdx.x;
   int tile, part;
   uint32_t cnt, off;  // the tile's bin entries: number and first
   int band_lo = 0, band_hi = TH - 1;  // tile rows of this work item
@@ -970,7 +993,7 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
     const int64_t t = b - nparts_all;
     const int64_t nlisted = (int64_t)min(a.counters[kListed], (u64)a.ntiles);
     if (t >= nlisted) return;
-    const uint4 le = a.tile_list[xcd_remap(t, nlisted)];
+    const uint4 le = a.tile_list[remap ? xcd_remap(t, nlisted) : t];
     tile = (int)le.x;
     part = 0;
     off = le.y;
@@ -1112,6 +1135,11 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
       }
     }
   }
+}
+
+template <int TW, int TH, int NT>
+__global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
+  stage_tile<TW, TH, NT>(a, blockIdx.x, true);
 }
 
 // Range image of the pixel from its depth (Renderer.cpp:388-415).
@@ -1361,10 +1389,12 @@ __device__ void reduce_stats(const BatchArgs& a, int tid) {
 // one job (each wavefront one kShadeW-pixel row segment).  Held to 7 waves
 // per SIMD (72 VGPRs, a few bytes of spill): the kernel is bound by its
 // chain of dependent loads, so more waves in flight pay off; 8 spill too much.
-__global__ __launch_bounds__(kShadeBlock, 7) void k_shade(BatchArgs a) {
+// vb: the workgroup's index in the grid of shade_blocks workgroups; remap:
+// blocks in XCD-interleaved order (own launch), else b = vb.
+__device__ __forceinline__ void stage_shade(const BatchArgs& a, int vb, bool remap) {
   __shared__ int64_t dcache[kMaxLdsDraws];
   const int tid = threadIdx.x;
-  const int b = (int)xcd_remap(blockIdx.x, a.shade_blocks);
+  const int b = remap ? (int)xcd_remap(vb, a.shade_blocks) : vb;
   const int j = __builtin_amdgcn_readfirstlane(
       job_lookup(a.shade_hint, a.job_shade_base, a.njobs, b));
   const JobDesc J = uniform_job(a, j);
@@ -1390,7 +1420,7 @@ __global__ __launch_bounds__(kShadeBlock, 7) void k_shade(BatchArgs a) {
   const int ncached = one_draw ? -1 : fill_draw_cache(dcache, a, J, tid, kShadeBlock);
   // The raster and scan are done with the tile counters, part counters and
   // look-back words: leave them zero for the next batch on this scratch.
-  for (int64_t i = (int64_t)blockIdx.x * kShadeBlock + tid; i < a.ntiles || i < a.scan_blocks;
+  for (int64_t i = (int64_t)vb * kShadeBlock + tid; i < a.ntiles || i < a.scan_blocks;
        i += (int64_t)a.shade_blocks * kShadeBlock) {
     if (i < a.ntiles) {
       a.tile_count[(size_t)i * kCtrStride64] = 0;
@@ -1399,11 +1429,260 @@ __global__ __launch_bounds__(kShadeBlock, 7) void k_shade(BatchArgs a) {
     if (i < a.scan_blocks) a.scan_status[i] = 0;
   }
   if (!one_draw) __syncthreads();
-  if (blockIdx.x == 0) reduce_stats(a, tid);
+  if (vb == 0) reduce_stats(a, tid);
   // After a bin-list or part-list overflow some tiles were not rastered (their
   // winner ids are stale): nothing is shaded, the batch is re-run.
   const bool overflow = a.counters[kBinOverflow] != 0ull || a.counters[kHeavyOverflow] != 0ull;
   if (inside && !overflow) shade_pixel(a, J, dcache, ncached, id, c, row);
+}
+
+__global__ __launch_bounds__(kShadeBlock, 7) void k_shade(BatchArgs a) {
+  stage_shade(a, blockIdx.x, true);
+}
+
+// ---- the persistent kernel --------------------------------------------------
+//
+// One launch of a resident grid runs every stage of the batch, each stage as
+// the same workgroups (vb, nvb) its own launch would run.  Workgroup w serves
+// queue x = w % kPQueues (with round-robin dispatch, one XCD per queue).
+// Stage s's workgroups are split into kPQueues contiguous ranges, one per
+// queue, and dealt out in items of consecutive workgroups by an atomic head
+// per (stage, queue).  A stage that waits for earlier ones starts once each
+// of them is complete: a workgroup leaving stage s adds the items it ran to
+// done[s][x] (after a release fence); the one completing queue x's count
+// bumps fin[s], and the one completing fin[s] sets flag[s][y] for every
+// queue y, which queue y's workgroups poll (one load per poll, bounded by a
+// wall-clock limit).  Items are only ever waited on by items of later stages,
+// and each queue's items are taken in order, so the launch drains while at
+// least one workgroup of each queue runs.  A wait that hits the limit sets
+// the error word: from then on workgroups skip the work of their items (still
+// counting them), the launch drains, and the error reaches the host through
+// the batch counters.  The last workgroup to finish clears the control words.
+
+// Polls are relaxed loads: an acquire at agent scope invalidates the L2 of
+// the XCD, and workgroups out of items poll while the others still work on
+// the stage.  The one acquire fence after a wait orders what follows.
+__device__ inline u64 ctl_load(const u64* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Stage workgroups of the batch; the raster's from the scan's counts (the
+// grid of its own launch is a bound, whose workgroups past the count exit).
+__device__ inline int64_t persist_stage_size(const BatchArgs& a, const PersistPlan& q, int s) {
+  if (s != kPTile) return q.nvb[s];
+  const int64_t nparts_all =
+      a.counters[kHeavyOverflow] ? 0 : (int64_t)min(a.counters[kParts], (u64)a.parts_cap);
+  const int64_t nlisted = (int64_t)min(a.counters[kListed], (u64)a.ntiles);
+  return min((int64_t)q.nvb[s], nparts_all + nlisted);
+}
+
+// Queue x's share of a stage of n workgroups: first workgroup, count, and
+// workgroups per item.
+struct QueueShare {
+  int64_t lo, len, per, items;
+};
+__device__ inline QueueShare queue_share(const PersistPlan& q, int s, int64_t n, int x, int G) {
+  QueueShare r;
+  r.lo = n * x / kPQueues;
+  r.len = n * (x + 1) / kPQueues - r.lo;
+  const int64_t wgs = G / kPQueues + (x < G % kPQueues ? 1 : 0);
+  const int64_t want = max((int64_t)1, wgs * (int64_t)q.depth[s]);
+  r.per = max((int64_t)1, (r.len + want - 1) / want);
+  r.items = (r.len + r.per - 1) / r.per;
+  return r;
+}
+
+// The persistent kernel's one argument.
+struct PersistArgs {
+  BatchArgs a;
+  PersistPlan q;
+};
+
+// The kernel's argument in the kernel-argument segment, where it is the
+// first and only explicit argument (offset 0).  The kernel takes the
+// segment's address (only a kernel can: in a called function the intrinsic
+// yields null) and hands it to the stage functions.  The argument itself is
+// never named: taking its address makes the compiler copy it to private
+// memory, which is not in the kernel-argument segment.
+using KernargPersist = const __attribute__((address_space(4))) PersistArgs;
+
+__device__ __forceinline__ KernargPersist* persist_kernarg() {
+  return (KernargPersist*)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
+// The argument through a pointer the compiler cannot see through: loads
+// after it are not hoisted out of the loops around a stage call, where they
+// would be live across the whole stage (and spill).  Loads through it stay
+// scalar (constant address space).
+__device__ __forceinline__ const PersistArgs& persist_args(KernargPersist* pk) {
+  asm volatile("" : "+s"(pk));
+  return *(const PersistArgs*)pk;
+}
+
+template <int TW>
+__device__ __forceinline__ void persist_run(const BatchArgs& a, const PersistPlan& q, int s,
+                                            int64_t vb) {
+  switch (s) {
+    case kPClear: stage_clear(a, (int)vb, q.nvb[s]); break;
+    case kPVertex: stage_vertex(a, (int)vb, q.nvb[s]); break;
+    case kPGeom: stage_geom(a, (int)vb, q.nvb[s]); break;
+    case kPScan: stage_scan(a); break;
+    case kPBin: stage_bin(a, (int)vb, q.nvb[s]); break;
+    case kPRows: stage_rows(a, (int)vb, q.nvb[s]); break;
+    case kPTile: stage_tile<TW, TW, 256>(a, vb, false); break;
+    default: stage_shade(a, (int)vb, false); break;
+  }
+}
+
+// One stage of the persistent kernel for this workgroup: take its queue's
+// items until none is left, run their workgroups, then report the items run
+// (release, count, completion flags).  A function of its own per stage, not
+// inlined: each stage's registers are allocated for it alone (inlined
+// together, the stages spill), and the call is made once per stage.
+template <int TW, int S>
+__device__ __noinline__ void persist_stage(KernargPersist* pk_arg, bool err) {
+  // The pointer arrives in vector registers (a function argument): it is
+  // the same in every lane, which the compiler is told here.
+  const uint64_t pv = (uint64_t)(uintptr_t)pk_arg;
+  KernargPersist* const pk = (KernargPersist*)(uintptr_t)(
+      ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(pv >> 32)) << 32) |
+      (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pv));
+  __shared__ int64_t sh_item[2];
+  __shared__ int64_t sh_share[4];  // the stage's lo, len, per, items for this queue
+  const int tid = threadIdx.x;
+  const int G = gridDim.x;
+  const int x = blockIdx.x % kPQueues;
+  int64_t n;
+  {
+    const PersistArgs& pa = persist_args(pk);
+    n = persist_stage_size(pa.a, pa.q, S);
+  }
+  if (n == 0) return;
+  const u64 t_begin = persist_args(pk).q.stamps ? wall_clock64() : 0ull;
+  __syncthreads();  // every thread has read the previous stage's last item index
+  if (tid == 0) {
+    const PersistArgs& pa = persist_args(pk);
+    const QueueShare sh = queue_share(pa.q, S, n, x, G);
+    sh_share[0] = sh.lo;
+    sh_share[1] = sh.len;
+    sh_share[2] = sh.per;
+    sh_share[3] = sh.items;
+    sh_item[0] = (int64_t)atomicAdd(pa.q.ctl + (size_t)(kCtlHead + S * kPQueues + x) * kCtlStride,
+                                    1ull);
+  }
+  __syncthreads();
+  int64_t it = sh_item[0];
+  int par = 0;
+  u64 ran = 0;
+  while (it < sh_share[3]) {
+    u64 next = 0;
+    if (tid == 0)  // the next item, in flight during this one
+      next = atomicAdd(persist_args(pk).q.ctl + (size_t)(kCtlHead + S * kPQueues + x) * kCtlStride,
+                       1ull);
+    if (!err) {
+      const int64_t lo = sh_share[0], per = sh_share[2];
+      const int64_t b0 = lo + it * per;
+      const int64_t b1 = min(b0 + per, lo + sh_share[1]);
+      for (int64_t vb = b0; vb < b1; vb++) {
+        const PersistArgs& pa = persist_args(pk);
+        persist_run<TW>(pa.a, pa.q, S, vb);
+        __syncthreads();  // LDS of this workgroup is reused by the next
+      }
+    }
+    ran++;
+    par ^= 1;
+    if (tid == 0) sh_item[par] = (int64_t)next;
+    __syncthreads();
+    it = sh_item[par];
+  }
+  if (!ran) return;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (tid == 0) {
+    const PersistPlan& q = persist_args(pk).q;
+    if (q.stamps) {
+      atomicMax(q.stamps + (size_t)(2 * S) * kStampStride, ~t_begin);
+      atomicMax(q.stamps + (size_t)(2 * S + 1) * kStampStride, wall_clock64());
+    }
+    u64* const ctl = q.ctl;
+    const u64 old = __hip_atomic_fetch_add(ctl + (size_t)(kCtlDone + S * kPQueues + x) * kCtlStride,
+                                           ran, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old + ran == (u64)sh_share[3]) {
+      int nonempty = 0;
+      for (int y = 0; y < kPQueues; y++) nonempty += queue_share(q, S, n, y, G).items > 0;
+      const u64 f = __hip_atomic_fetch_add(ctl + (size_t)(kCtlFin + S) * kCtlStride, 1ull,
+                                           __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (f + 1 == (u64)nonempty)
+        for (int y = 0; y < kPQueues; y++)
+          __hip_atomic_store(ctl + (size_t)(kCtlFlag + S * kPQueues + y) * kCtlStride, 1ull,
+                             __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+template <int TW>
+__global__ __launch_bounds__(256, 6) void k_persist(PersistArgs) {
+  KernargPersist* const pk = persist_kernarg();
+  __shared__ int sh_err;
+  const int tid = threadIdx.x;
+  bool err = false;
+  int confirmed = -1;  // stages up to this one are known complete
+  for (int s = 0; s < kPStages; s++) {
+    const PersistPlan& q = persist_args(pk).q;
+    const BatchArgs& a = persist_args(pk).a;
+    const int x = blockIdx.x % kPQueues;
+    u64* const ctl = q.ctl;
+    auto word = [&](int i) -> u64* { return ctl + (size_t)i * kCtlStride; };
+    if (q.nvb[s] == 0) continue;
+    if (q.wait[s]) {
+      // Wait for every earlier stage that has items (thread 0 polls).
+      if (tid == 0) {
+        for (int t = confirmed + 1; t < s && !err; t++) {
+          if (q.nvb[t] == 0 || persist_stage_size(a, q, t) == 0) continue;
+          const u64 t0 = wall_clock64();
+          while (ctl_load(word(kCtlFlag + t * kPQueues + x)) == 0ull) {
+            if (ctl_load(word(kCtlErr)) != 0ull) {
+              err = true;
+              break;
+            }
+            if (wall_clock64() - t0 > q.time_limit) {
+              __hip_atomic_store(word(kCtlErr), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              err = true;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(8);
+          }
+        }
+        sh_err = err;
+      }
+      __syncthreads();
+      err = sh_err != 0;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      confirmed = s - 1;
+    }
+    switch (s) {
+      case kPClear: persist_stage<TW, kPClear>(pk, err); break;
+      case kPVertex: persist_stage<TW, kPVertex>(pk, err); break;
+      case kPGeom: persist_stage<TW, kPGeom>(pk, err); break;
+      case kPScan: persist_stage<TW, kPScan>(pk, err); break;
+      case kPBin: persist_stage<TW, kPBin>(pk, err); break;
+      case kPRows: persist_stage<TW, kPRows>(pk, err); break;
+      case kPTile: persist_stage<TW, kPTile>(pk, err); break;
+      default: persist_stage<TW, kPShade>(pk, err); break;
+    }
+  }
+  if (tid == 0) {
+    u64* const ctl = persist_args(pk).q.ctl;
+    const u64 old = __hip_atomic_fetch_add(ctl + (size_t)kCtlExit * kCtlStride, 1ull,
+                                           __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old + 1 == (u64)gridDim.x) {  // every other workgroup is done with the words
+      if (ctl_load(ctl + (size_t)kCtlErr * kCtlStride))
+        atomicOr(&persist_args(pk).a.counters[kError], 1ull);
+      for (int i = 0; i < kCtlWords; i++)
+        __hip_atomic_store(ctl + (size_t)i * kCtlStride, 0ull, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
 
 // ---- scratch of the entry point that allocates nothing ----------------------
@@ -1447,10 +1726,19 @@ void* noalloc_scratch(int device) {
 
 // ---- launchers -------------------------------------------------------------
 
-hipError_t launch_clear(const BatchArgs& a, hipStream_t s) {
+static int clear_grid(const BatchArgs& a) {
   const int n = std::max(std::max(a.scan_blocks, a.ntiles), (int)kNumCounters);
-  const int grid = std::min((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_clear, dim3(grid), dim3(256), 0, s, a);
+  return std::min((n + 255) / 256, 4096);
+}
+
+static int vertex_grid(const BatchArgs& a) {
+  if (a.total_verts == 0) return 0;
+  const int64_t blocks = (a.total_verts + kVertexStep - 1) / kVertexStep;
+  return (int)std::min<int64_t>(blocks, (int64_t)a.num_cus * 8);
+}
+
+hipError_t launch_clear(const BatchArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear, dim3(clear_grid(a)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -1468,9 +1756,7 @@ int geom_blocks_per_cu() {
 
 hipError_t launch_vertex(const BatchArgs& a, hipStream_t s) {
   if (a.total_verts == 0) return hipSuccess;
-  const int64_t blocks = (a.total_verts + kVertexStep - 1) / kVertexStep;
-  const int grid = (int)std::min<int64_t>(blocks, (int64_t)a.num_cus * 8);
-  hipLaunchKernelGGL(k_vertex, dim3(grid), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_vertex, dim3(vertex_grid(a)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -1536,6 +1822,73 @@ hipError_t launch_tile(const BatchArgs& a, hipStream_t s) {
 hipError_t launch_shade(const BatchArgs& a, hipStream_t s) {
   const int grid = a.shade_blocks;
   hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kShadeBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+// 16 x 16 tiles only: with 32 x 32 tiles the kernel's registers leave two
+// workgroups per CU.
+bool persistent_supported(int tw, int th) { return tw == 16 && th == 16; }
+
+// Resident workgroups per CU of the persistent kernel, and the wall clock's
+// ticks per second (device 0's figures stand for every device: one GPU model).
+template <int TW>
+static int persist_blocks_per_cu() {
+  static const int n = [] {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_persist<TW>, 256, 0) != hipSuccess ||
+        b < 1) {
+      (void)hipGetLastError();
+      b = 1;
+    }
+    return b;
+  }();
+  return n;
+}
+
+unsigned long long wall_clock_hz() {
+  static const unsigned long long hz = [] {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+        khz <= 0) {
+      (void)hipGetLastError();
+      khz = 100000;  // 100 MHz
+    }
+    return (unsigned long long)khz * 1000ull;
+  }();
+  return hz;
+}
+
+hipError_t launch_persistent(const BatchArgs& a, bool clear, unsigned long long* ctl,
+                             unsigned long long* stamps, hipStream_t s) {
+  if (!persistent_supported(a.tile_w, a.tile_h) || !ctl) return hipErrorInvalidValue;
+  PersistPlan q;
+  memset(&q, 0, sizeof(q));
+  q.nvb[kPClear] = clear ? clear_grid(a) : 0;
+  q.nvb[kPVertex] = vertex_grid(a);
+  q.nvb[kPGeom] = a.total_tris ? geom_grid(a) : 0;
+  q.nvb[kPScan] = a.scan_blocks;
+  q.nvb[kPBin] = bin_grid(a);
+  q.nvb[kPRows] = bin_grid(a);
+  q.nvb[kPTile] = (int32_t)std::min<int64_t>((int64_t)a.ntiles + (int64_t)a.parts_cap, INT32_MAX);
+  q.nvb[kPShade] = a.shade_blocks;
+  // Items per queue workgroup: one for the even stages, several for the
+  // raster and shading, whose workgroups differ in cost; the scan's
+  // workgroups one at a time (each waits on its predecessors).
+  const int32_t depth[kPStages] = {1, 1, 1, 1 << 30, 1, 1, 8, 4};
+  const int32_t wait[kPStages] = {0, 0, 1, 1, 1, 1, 1, 1};
+  for (int i = 0; i < kPStages; i++) {
+    q.depth[i] = depth[i];
+    q.wait[i] = wait[i];
+  }
+  q.ctl = ctl;
+  q.stamps = stamps;
+  q.time_limit = 2ull * wall_clock_hz();  // 2 s per wait
+  const int grid = std::max(persist_blocks_per_cu<16>() * std::max(a.num_cus, 1), kPQueues);
+  PersistArgs pa;
+  pa.a = a;
+  pa.q = q;
+  hipLaunchKernelGGL(k_persist<16>, dim3(grid), dim3(256), 0, s, pa);
   return hipGetLastError();
 }
 

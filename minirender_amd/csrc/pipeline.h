@@ -123,6 +123,47 @@ constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
 constexpr int kMaxDevices = 64;
 int noalloc_scratch_init(int device);
 void* noalloc_scratch(int device);
+// ---- the persistent kernel: every stage of a batch in one launch ----------
+// Stages in queue order.
+enum PersistStage : int32_t {
+  kPClear, kPVertex, kPGeom, kPScan, kPBin, kPRows, kPTile, kPShade, kPStages
+};
+constexpr int kPQueues = 8;      // work queues (one per XCD under round-robin dispatch)
+constexpr int kCtlStride = 32;   // u64 words between two control words (256 B)
+// Control words (each kCtlStride words apart): queue heads and completed
+// items per (stage, queue), queues completed per stage, completion flags per
+// (stage, queue), workgroups finished, error.  Zero between launches: the
+// last workgroup of a launch clears them.
+constexpr int kCtlHead = 0;
+constexpr int kCtlDone = kCtlHead + kPStages * kPQueues;
+constexpr int kCtlFin = kCtlDone + kPStages * kPQueues;
+constexpr int kCtlFlag = kCtlFin + kPStages;
+constexpr int kCtlExit = kCtlFlag + kPStages * kPQueues;
+constexpr int kCtlErr = kCtlExit + 1;
+constexpr int kCtlWords = kCtlErr + 1;
+constexpr size_t kCtlBytes = (size_t)kCtlWords * kCtlStride * 8;
+
+struct PersistPlan {
+  int32_t nvb[kPStages];    // workgroups of the stage's own launch (0: not run)
+  int32_t depth[kPStages];  // items per workgroup a stage is dealt out in (>= 1)
+  int32_t wait[kPStages];   // 1: the stage waits until every earlier stage is complete
+  unsigned long long* ctl;  // kCtlBytes of control words, zero before the launch
+  unsigned long long time_limit;  // wall-clock ticks one wait may take
+  // Profiling (null: off): per stage s, words kStampStride * (2 s) and
+  // kStampStride * (2 s + 1) take the maximum over workgroups of ~(first
+  // wall-clock tick of the stage) and of its last tick; zero before the launch.
+  unsigned long long* stamps;
+};
+constexpr int kStampStride = 32;
+constexpr size_t kStampBytes = (size_t)2 * kPStages * kStampStride * 8;
+// Launch the persistent kernel for a batch (16 x 16 tiles), on a grid of the
+// workgroups the GPU holds at once.  `clear`: run the clear stage.
+hipError_t launch_persistent(const BatchArgs& a, bool clear, unsigned long long* ctl,
+                             unsigned long long* stamps, hipStream_t s);
+// Wall-clock ticks per second of the device's clock (wall_clock64).
+unsigned long long wall_clock_hz();
+bool persistent_supported(int tile_w, int tile_h);
+
 bool tile_supported(int tile_w, int tile_h);
 // Raster workgroup sizes built per tile size: 8x8 {64, 128}, 16x16 and
 // 32x32 {64, 128, 256, 512}; the default is 64 for 8x8, else 256.
