@@ -16,15 +16,19 @@
 //             the others with pairs listed for the raster, empty ones marked
 //   k_bin     (c) scatter of (triangle id, draw) into the tiles' bin lists:
 //             per wavefront step, one atomic instruction for all distinct
-//             tiles; takes the edge checkpoints of triangles spanning
-//             several tile columns
-//   k_tile    (d) per-tile raster (listed tiles and parts) with the tile's 64-bit
-//             (depth key, draw-order id) buffer in LDS and ds_min_u64; parts
-//             of a split tile merge through global 64-bit atomicMin.  Writes
-//             each pixel's winning id and the rows' edge checkpoints
+//             tiles; allocates the edge checkpoints and row tasks of
+//             triangles at least 64 pixels wide spanning several tile columns
+//   k_rows    (c) row pre-pass: one lane per row of such a triangle walks it
+//             once and stores the edge values where it enters each tile column
+//   k_tile    (d) per-tile raster (listed tiles, parts of split tiles, row
+//             bands) with the tile's 64-bit (depth key, draw-order id) buffer
+//             in LDS and ds_min_u64; parts of a split tile merge through
+//             global 64-bit atomicMin.  Writes each pixel's winning id
 //   k_shade   (e) deferred Blinn-Phong + texture, one lane per pixel in image
 //             order, writing RGB8 / depth / winner / float colour / normals /
 //             range image (background for the pixels of empty tiles)
+//   k_persist every stage above in one launch of a resident grid fed by
+//             device work queues (optional; the same stage functions)
 //
 // A triangle's id is 2 * (flat triangle index in the batch) + clip sub-index,
 // which increases with draw order inside a job.  The minimum of
@@ -1214,8 +1218,8 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   TriSetup s;
   tri_setup_corners(J, ec, vc, sub, D.no_clip != 0, s);
   // Edge values at the pixel, from the same serial walk along the row as the
-  // raster's: from the raster's checkpoint where the row enters the pixel's
-  // tile column, or from the bbox's left column.
+  // raster's: from the row pre-pass's checkpoint where the row enters the
+  // pixel's tile column, or from the bbox's left column.
   float e1, e2;
   if (cb.y && ctx > (s.bx0 >> sw)) {
     e1 = st.x;

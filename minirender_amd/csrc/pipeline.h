@@ -28,7 +28,8 @@ struct BatchArgs {
   uint64_t bins_cap;
   // Edge checkpoints: the edge values of each row of a triangle spanning
   // several tile columns where the row enters each of them after the first
-  // (written by the raster, read by shading).  Checkpoint of (row, tile
+  // (written by the row pre-pass, read by the raster and shading).
+  // Checkpoint of (row, tile
   // column tx) of a survivor id: ckpt[ckpt_base[id].x + row *
   // ckpt_base[id].y + tx], with .x a signed offset (the entries of the first
   // column are allocated, never written); .y == 0: no checkpoints.
