@@ -569,16 +569,22 @@ def serve_study(args, torch, mra, reqs, graphs=True):
         ctx.set_bin_factor(cfg["bin_factor"])
         max_batch, slots, use_graphs = modes[mode]
         ctx.set_graphs(use_graphs)
+        g0 = ctx.stats()
         try:
             with mra.Server(ctx, max_batch=max_batch, slots=slots,
                             hbm_budget=32 * (1 << 30)) as srv:
                 lat, mk = srv.run_trace(farr, list(arr[:count]), oarr[where])
         finally:
             ctx.set_graphs(False)
-        return {"p50_ms": round(float(np.percentile(lat, 50)), 3),
-                "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                "frames_per_s": round(count / mk, 1), "requests": count,
-                "offered_rate": round(float(count / max(arr[count - 1], 1e-9)), 1)}
+        g1 = ctx.stats()
+        r = {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+             "p99_ms": round(float(np.percentile(lat, 99)), 3),
+             "frames_per_s": round(count / mk, 1), "requests": count,
+             "offered_rate": round(float(count / max(arr[count - 1], 1e-9)), 1)}
+        if use_graphs:  # batches replayed as captured, after an in-place update, new graphs
+            r["graphs"] = {k: int(getattr(g1, "graph_" + k) - getattr(g0, "graph_" + k))
+                           for k in ("replays", "updates", "captures")}
+        return r
 
     trace("batched", np.zeros(n), min(n, 1024), "device")  # warm
     out = {"rate": args.serve_rate, "hbm_budget_gb": 32, "p99_budget_ms": args.p99_budget,

@@ -107,6 +107,10 @@ typedef struct mr_stats {
   double frame_ms;            /* first to last kernel of the frame (profiling)  */
   int64_t launches;           /* kernel launches of the last batch (1: the     */
                               /* persistent kernel)                             */
+  /* Captured batches (mr_ctx_set_graphs) over every slot of the context     */
+  /* since its creation: replayed as captured, replayed after updating an    */
+  /* instantiated graph of the same kernels in place, newly instantiated.    */
+  int64_t graph_replays, graph_updates, graph_captures;
 } mr_stats;
 
 /* ---- device context: HIP device, stream, resident meshes/textures ------- */
@@ -167,9 +171,12 @@ int mr_ctx_set_band_work(mr_ctx* ctx, int32_t units);
  * (>= 1); overflow re-runs the frame with a larger list. */
 int mr_ctx_set_bin_factor(mr_ctx* ctx, float entries_per_triangle);
 /* Capture the steady state as HIP graphs: a batch whose launch parameters
- * (sizes, scratch, tile configuration) equal the previous one on the same
+ * (sizes, scratch, tile configuration) equal an earlier one on the same
  * slot replays the captured uploads + kernels instead of launching them one
- * by one.  Off by default; ignored while profiling or on an external stream.
+ * by one; a batch of other sizes with the same kernels updates such a graph
+ * in place (hipGraphExecUpdate) and replays it, so a service of mixed
+ * batches reuses a few instantiated graphs (mr_stats.graph_*).  Off by
+ * default; ignored while profiling or on an external stream.
  * On MI355X with ROCm 7.2 a replayed single frame measured slower than eager
  * launches (which overlap the GPU's work); it is for callers whose host
  * thread is the bottleneck.  No counterpart in the reference (host-side

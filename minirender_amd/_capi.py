@@ -64,6 +64,7 @@ class mr_stats(C.Structure):
         ("records", C.c_int64), ("bin_entries", C.c_int64), ("tiles", C.c_int64),
         ("retries", C.c_int64), ("stage_ms", C.c_double * 8), ("frame_ms", C.c_double),
         ("launches", C.c_int64),
+        ("graph_replays", C.c_int64), ("graph_updates", C.c_int64), ("graph_captures", C.c_int64),
     ]
 
 

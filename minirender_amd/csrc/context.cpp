@@ -523,6 +523,14 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
         hit = i;
         break;
       }
+    // A batch of other sizes with the same kernels (same clear, vertex and
+    // geometry launches, raster template, persistent or not) updates the
+    // parameters of an instantiated graph in place instead of instantiating
+    // a new one, so a service of mixed batches keeps replaying a few graphs.
+    const int32_t topo_v[6] = {clear ? 1 : 0, a.total_verts ? 1 : 0, a.total_tris ? 1 : 0,
+                               a.tile_w, a.raster_threads, persistent ? 1 : 0};
+    std::vector<unsigned char> topo((const unsigned char*)topo_v,
+                                    (const unsigned char*)topo_v + sizeof(topo_v));
     if (hit == s.graphs.size()) {
       HIP_TRY(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
       const int r = issue();
@@ -533,20 +541,42 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
         return r;
       }
       if (e != hipSuccess) return MR_ERR_HIP;
-      Slot::Graph ng;
-      const hipError_t ei = hipGraphInstantiate(&ng.exec, g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
-      if (ei != hipSuccess) return MR_ERR_HIP;
-      ng.key = std::move(key);
-      if (s.graphs.size() >= kGraphCache) {
-        hipGraphExecDestroy(s.graphs.back().exec);
-        s.graphs.pop_back();
+      size_t upd = s.graphs.size();
+      for (size_t i = 0; i < s.graphs.size(); i++) {
+        if (s.graphs[i].topo != topo) continue;
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(s.graphs[i].exec, g, &err_node, &res) == hipSuccess &&
+            res == hipGraphExecUpdateSuccess) {
+          upd = i;
+          break;
+        }
+        (void)hipGetLastError();
       }
-      s.graphs.insert(s.graphs.begin(), std::move(ng));
-      s.graph_captures++;
+      if (upd < s.graphs.size()) {
+        hipGraphDestroy(g);
+        s.graphs[upd].key = std::move(key);
+        if (upd) std::rotate(s.graphs.begin(), s.graphs.begin() + upd, s.graphs.begin() + upd + 1);
+        c->graph_updates++;
+      } else {
+        Slot::Graph ng;
+        const hipError_t ei = hipGraphInstantiate(&ng.exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) return MR_ERR_HIP;
+        ng.key = std::move(key);
+        ng.topo = std::move(topo);
+        if (s.graphs.size() >= kGraphCache) {
+          hipGraphExecDestroy(s.graphs.back().exec);
+          s.graphs.pop_back();
+        }
+        s.graphs.insert(s.graphs.begin(), std::move(ng));
+        s.graph_captures++;
+        c->graph_captures++;
+      }
     } else {
       if (hit) std::rotate(s.graphs.begin(), s.graphs.begin() + hit, s.graphs.begin() + hit + 1);
       s.graph_replays++;
+      c->graph_replays++;
     }
     HIP_TRY(hipGraphLaunch(s.graphs.front().exec, s.stream));
   } else {
@@ -918,6 +948,9 @@ int mr_ctx_get_stats(mr_ctx* c, mr_stats* st) {
   if (!c || !st) return MR_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   *st = c->stats;
+  st->graph_replays = c->graph_replays;
+  st->graph_updates = c->graph_updates;
+  st->graph_captures = c->graph_captures;
   return MR_OK;
 }
 

@@ -106,7 +106,8 @@ struct Slot {
   // instantiated graphs, keyed by the launch parameters, most recent first
   // (at most kGraphCache); a batch with a known key replays its graph.
   struct Graph {
-    std::vector<unsigned char> key;
+    std::vector<unsigned char> key;   // launch parameters it was captured (or updated) with
+    std::vector<unsigned char> topo;  // what fixes its nodes: kernels, template, copies
     hipGraphExec_t exec = nullptr;
   };
   std::vector<Graph> graphs;
@@ -135,6 +136,8 @@ struct mr_ctx {
   bool profiling = false;
   bool graphs = false;  // capture steady-state batches as HIP graphs
   bool persistent = false;  // run batches as one persistent launch (16 x 16 tiles)
+  // Graph use over all slots of the context (captured batches).
+  int64_t graph_replays = 0, graph_updates = 0, graph_captures = 0;
   mr_stats stats{};
   int32_t num_cus = 256;
   std::recursive_mutex mu;

@@ -242,6 +242,35 @@ def test_graph_replay_matches_eager(gpu_ctx):
     assert_parity(eager[2], oracle_py.render(views[2], ALL), "graph view 2", normals=False)
 
 
+def test_graph_update_mixed_shapes(gpu_ctx):
+    """Captured batches of different sizes with the same kernels update one
+    instantiated graph in place and replay it: every result equals the
+    eager render, across three rounds with eager work in between, and the
+    context counts updates and replays rather than new instantiations."""
+    import minirender_amd as mra
+    frames = [fuzz_frame(600 + i, 64 + 24 * i, 48 + 10 * i) for i in range(5)]
+    batches = [frames[:2], frames[2:5], frames[1:4]]
+    kinds = ("rgb8", "depth", "winner", "rgb")
+    eager = [gpu_ctx.render_batch(b, kinds) for b in batches]
+    ctx = mra.Context(0)
+    try:
+        ctx.set_graphs(True)
+        for rep in range(3):
+            for bi, b in enumerate(batches):
+                out = ctx.render_batch(b, kinds)
+                for j in range(len(b)):
+                    for k in kinds:
+                        assert np.array_equal(out[j][k].view(np.uint8),
+                                              eager[bi][j][k].view(np.uint8)), (rep, bi, j, k)
+            gpu_ctx.render(frames[0], ("rgb8",))  # eager work on another context
+        st = ctx.stats()
+    finally:
+        ctx.close()
+    counts = (st.graph_replays, st.graph_updates, st.graph_captures)
+    assert sum(counts) == 9, counts  # every batch went through a graph
+    assert st.graph_captures <= 3, counts
+
+
 def test_autotune_and_cache(tmp_path):
     """The autotuner picks a built configuration, results stay exact under
     it, and a second call with the same cache file reads the choice back."""
