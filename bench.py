@@ -293,19 +293,22 @@ def plain_section(args, torch, mra, frame, wl_name):
 def cpu_section(args, frame, wl):
     from oracle import oracle_py
     budget = args.cpu_seconds
-    oracle_py.render(frame, ("rgb8", "depth"))  # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
+    for _ in range(2):  # warm-up
         oracle_py.render(frame, ("rgb8", "depth"))
-        n += 1
+    times, t0 = [], time.perf_counter()
+    while True:
+        t1 = time.perf_counter()
+        oracle_py.render(frame, ("rgb8", "depth"))
+        times.append(time.perf_counter() - t1)
         el = time.perf_counter() - t0
-        if el >= budget or (n >= 2 and el * (n + 1) / n > 3 * budget):
+        if (el >= budget and len(times) >= 10) or el >= 3 * budget:
             break
-    per = el / n
+    per = float(np.median(times))
     res = {"value": round(frame.ntri / per / 1e6, 3), "unit": "Mtris/s", "cores": 1,
            "kind": "port", "frames_per_s": round(1.0 / per, 4), "host": host_cpu(),
-           "sample": f"{n} frames of the bench workload ({wl}) after 1 warm-up frame, "
-                     f"{el:.1f} s, one thread, oracle/liboracle.so (-O3 -ffp-contract=off)"}
+           "sample": f"median of {len(times)} frames of the bench workload ({wl}) after 2 "
+                     f"warm-up frames, {el:.1f} s, one thread, oracle/liboracle.so "
+                     f"(-O3 -ffp-contract=off)"}
     return res
 
 
