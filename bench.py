@@ -211,7 +211,12 @@ def plain_section(args, torch, mra, frame, wl_name):
         ctx.set_band_work(args.band_work)
     if args.tile:
         ctx.set_tile(args.tile, args.tile)
-        tune = {"tile": args.tile, "fixed": True}
+        if args.raster_threads:
+            ctx.set_raster_threads(args.raster_threads)
+        if args.frag_size:
+            ctx.set_frag_size(args.frag_size)
+        tune = {"tile": args.tile, "raster_threads": args.raster_threads,
+                "frag_size": args.frag_size, "fixed": True}
     else:
         tune = ctx.autotune([frame], outs=[outs], reps=3, cache_path=args.tune_cache)
     tile = tune["tile"]
@@ -285,7 +290,8 @@ def plain_section(args, torch, mra, frame, wl_name):
     res = {"ms_per_step": dt * 1e3, "value": frame.ntri / dt / 1e6, "frames_per_s": 1.0 / dt,
            "tile": tile, "autotune": tune, "kernels": kernels, "roofline": roof,
            "records": st.records, "bin_entries": st.bin_entries,
-           "clipped": st.triangles_clipped, "retries": st.retries}
+           "clipped": st.triangles_clipped, "retries": st.retries,
+           "split_tiles": st.split_tiles, "raster_parts": st.raster_parts}
     ctx.close()
     return res
 
@@ -714,7 +720,8 @@ def main():
         "roofline": plain["roofline"],
         "kernels": plain["kernels"],
         "stats": {"records": plain["records"], "bin_entries": plain["bin_entries"],
-                  "clipped": plain["clipped"], "retries": plain["retries"]},
+                  "clipped": plain["clipped"], "retries": plain["retries"],
+                  "split_tiles": plain["split_tiles"], "raster_parts": plain["raster_parts"]},
         "cpu_baseline": cpu,
     }
     if not args.no_batched:

@@ -111,6 +111,9 @@ typedef struct mr_stats {
   /* since its creation: replayed as captured, replayed after updating an    */
   /* instantiated graph of the same kernels in place, newly instantiated.    */
   int64_t graph_replays, graph_updates, graph_captures;
+  int64_t split_tiles;        /* tiles split into parts of frag_size entries  */
+  int64_t raster_parts;       /* parts and row bands rastered by workgroups    */
+                              /* of their own                                   */
 } mr_stats;
 
 /* ---- device context: HIP device, stream, resident meshes/textures ------- */

@@ -65,6 +65,7 @@ class mr_stats(C.Structure):
         ("retries", C.c_int64), ("stage_ms", C.c_double * 8), ("frame_ms", C.c_double),
         ("launches", C.c_int64),
         ("graph_replays", C.c_int64), ("graph_updates", C.c_int64), ("graph_captures", C.c_int64),
+        ("split_tiles", C.c_int64), ("raster_parts", C.c_int64),
     ]
 
 

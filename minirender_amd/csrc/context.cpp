@@ -630,6 +630,8 @@ int finish_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
     st.tiles = p.ntiles;
     st.retries = s.retries;
     st.launches = s.last_launches;
+    st.split_tiles = (int64_t)hc[kHeavy];
+    st.raster_parts = (int64_t)hc[kParts];
     if (profile && s.last_launches == 1 && x.ctl.p) {
       // One persistent launch: each stage's span from its workgroups' stamps
       // (first start to last end; the stages of the vertex and geometry
