@@ -71,6 +71,8 @@ def parse():
                    help="autotuner cache file: reuse (or record) the launch configuration")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--no-configs", action="store_true",
+                   help="skip the plain path on the other single-frame configurations")
     p.add_argument("--no-serving", action="store_true")
     p.add_argument("--no-target", action="store_true",
                    help="skip the 1M-triangle 1080p service comparison")
@@ -294,6 +296,27 @@ def plain_section(args, torch, mra, frame, wl_name):
            "split_tiles": st.split_tiles, "raster_parts": st.raster_parts}
     ctx.close()
     return res
+
+
+def configs_section(args, torch, mra):
+    """The plain path on the other single-frame configurations: the sphere at
+    640x480 (the reference's CPU-runnable case), the textured height field at
+    1080p and the 1M-triangle grid at 1080p, each autotuned, with its frame
+    roofline (algorithmic bytes over the frame's GPU time)."""
+    out = {}
+    for name in ("sphere", "textured", "megatri"):
+        frame, wl = make_frame(name)
+        r = plain_section(args, torch, mra, frame, name)
+        out[name] = {"workload": wl, "triangles": frame.ntri,
+                     "ms_per_step": round(r["ms_per_step"], 4),
+                     "Mtris_per_s": round(r["value"], 1),
+                     "frames_per_s": round(r["frames_per_s"], 1),
+                     "gpu_ms": r["roofline"]["avg_launch_ms"],
+                     "roofline_frac": r["roofline"]["frac"],
+                     "alg_bytes": r["roofline"]["alg_bytes_per_launch"],
+                     "autotune": r["autotune"]}
+        log(f"config {name}: {out[name]}")
+    return out
 
 
 def cpu_section(args, frame, wl):
@@ -724,6 +747,8 @@ def main():
                   "split_tiles": plain["split_tiles"], "raster_parts": plain["raster_parts"]},
         "cpu_baseline": cpu,
     }
+    if not args.no_configs:
+        res["configs"] = configs_section(args, torch, mra)
     if not args.no_batched:
         res["batched"] = batched_section(args, torch, mra)
         log(f"batched: {res['batched']}")

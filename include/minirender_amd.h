@@ -202,6 +202,13 @@ int mr_ctx_set_graphs(mr_ctx* ctx, int32_t on);
  * Replaces the reference's per-frame call loop (samples/bench.cpp:128-135,
  * src/Renderer.cpp:311-332). */
 int mr_ctx_set_persistent(mr_ctx* ctx, int32_t on);
+
+/* How long one wait of the persistent kernel may take before the batch
+ * gives up with MR_ERR_DEVICE, in microseconds (negative: the default,
+ * 2 s).  0 makes every wait give up at once: the failure path can be
+ * tested from the host while every kernel still exits normally.  No
+ * counterpart in the reference. */
+int mr_ctx_set_wait_limit(mr_ctx* ctx, int64_t us);
 /* Record per-stage HIP event times into mr_stats (adds event overhead). */
 int mr_ctx_set_profiling(mr_ctx* ctx, int32_t on);
 int mr_ctx_get_stats(mr_ctx* ctx, mr_stats* stats);

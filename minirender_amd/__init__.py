@@ -155,6 +155,11 @@ class Context:
         """Synchronous batches as one launch of the persistent kernel."""
         check(self.lib.mr_ctx_set_persistent(self.h, int(on)), "mr_ctx_set_persistent")
 
+    def set_wait_limit(self, us: int) -> None:
+        """Microseconds one wait of the persistent kernel may take (negative:
+        default; 0: every wait gives up, the batch fails with MR_ERR_DEVICE)."""
+        check(self.lib.mr_ctx_set_wait_limit(self.h, int(us)), "mr_ctx_set_wait_limit")
+
     def set_stream(self, stream: int) -> None:
         check(self.lib.mr_ctx_set_stream(self.h, C.c_void_p(stream)), "mr_ctx_set_stream")
 

@@ -112,6 +112,7 @@ SIGNATURES = {
     "mr_ctx_set_profiling": (C.c_int, [_P, _I]),
     "mr_ctx_set_graphs": (C.c_int, [_P, _I]),
     "mr_ctx_set_persistent": (C.c_int, [_P, _I]),
+    "mr_ctx_set_wait_limit": (C.c_int, [_P, C.c_int64]),
     "mr_ctx_get_stats": (C.c_int, [_P, C.POINTER(mr_stats)]),
     "mr_ctx_memory": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "mr_mesh_upload": (C.c_int, [_P, _P, _I, _P, _I, _P, _I, _P, _P, _P, _I, _IP]),

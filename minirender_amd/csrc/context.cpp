@@ -417,14 +417,14 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
 // persistent: every stage in one launch of the persistent kernel (profiling
 // then times the whole launch as stage 0).
 static int launch_stages(const BatchArgs& a, Slot& s, bool profile, bool clear,
-                         bool persistent) {
+                         bool persistent, uint64_t wait_us) {
   hipStream_t st = s.stream;
   if (persistent) {
     unsigned long long* ctl = (unsigned long long*)s.sc.ctl.p;
     unsigned long long* stamps = profile ? ctl + kCtlBytes / 8 : nullptr;
     if (profile) HIP_TRY(hipMemsetAsync(stamps, 0, kStampBytes, st));
     if (profile) HIP_TRY(hipEventRecord(s.ev[0], st));
-    HIP_TRY(launch_persistent(a, clear, ctl, stamps, st));
+    HIP_TRY(launch_persistent(a, clear, ctl, stamps, wait_us, st));
     if (profile)
       for (int i = 1; i <= kStages; i++) HIP_TRY(hipEventRecord(s.ev[i], st));
     s.last_launches = 1;
@@ -503,7 +503,7 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   auto issue = [&]() -> int {
     HIP_TRY(hipMemcpyAsync(x.tables.p, x.h_tables.p, p.layout.bytes, hipMemcpyHostToDevice,
                            s.stream));
-    int r = launch_stages(a, s, profile, clear, persistent);
+    int r = launch_stages(a, s, profile, clear, persistent, c->wait_us);
     if (r) return r;
     HIP_TRY(hipMemcpyAsync(x.h_counters.p, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost,
                            s.stream));
@@ -598,8 +598,12 @@ int finish_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   for (int attempt = 0;; attempt++) {
     HIP_TRY(hipEventSynchronize(s.done));
     if (hc[kError]) {
-      // A bounded wait gave up.  The persistent kernel's last workgroup
-      // clears its control words in any case; cleared again to be sure.
+      // A bounded wait gave up: stages after it did not run, so the tile
+      // counters and look-back words shading re-zeroes may hold this batch's
+      // values; the next batch clears them first.  The persistent kernel's
+      // last workgroup clears its control words in any case; cleared again
+      // to be sure.
+      s.clean_tiles = s.clean_scan = 0;
       if (x.ctl.p) HIP_TRY(hipMemset(x.ctl.p, 0, kCtlBytes));
       return MR_ERR_DEVICE;
     }
@@ -781,7 +785,7 @@ static int render_noalloc(mr_ctx* c, const mr_frame* frame, const mr_outputs* ou
   const BatchArgs a = make_args(c, ns, p);
   unsigned long long* hc = g_na_counters[c->device];
   HIP_TRY(hipMemcpyAsync(ns.sc.tables.p, ht, p.layout.bytes, hipMemcpyHostToDevice, ns.stream));
-  if ((rc = launch_stages(a, ns, false, true, false))) return rc;
+  if ((rc = launch_stages(a, ns, false, true, false, 0))) return rc;
   HIP_TRY(hipMemcpyAsync(hc, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost, ns.stream));
   HIP_TRY(hipStreamSynchronize(ns.stream));
   if (hc[kError]) return MR_ERR_DEVICE;
@@ -928,6 +932,13 @@ int mr_ctx_set_bin_factor(mr_ctx* c, float f) {
 int mr_ctx_set_profiling(mr_ctx* c, int32_t on) {
   if (!c) return MR_ERR_ARG;
   c->profiling = on != 0;
+  return MR_OK;
+}
+
+int mr_ctx_set_wait_limit(mr_ctx* c, int64_t us) {
+  if (!c) return MR_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  c->wait_us = us < 0 ? 2000000ull : (uint64_t)us;
   return MR_OK;
 }
 

@@ -136,6 +136,7 @@ struct mr_ctx {
   bool profiling = false;
   bool graphs = false;  // capture steady-state batches as HIP graphs
   bool persistent = false;  // run batches as one persistent launch (16 x 16 tiles)
+  uint64_t wait_us = 2000000;  // how long one of its waits may take
   // Graph use over all slots of the context (captured batches).
   int64_t graph_replays = 0, graph_updates = 0, graph_captures = 0;
   mr_stats stats{};

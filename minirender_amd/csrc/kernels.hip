@@ -1639,8 +1639,13 @@ __global__ __launch_bounds__(256, 6) void k_persist(PersistArgs) {
     auto word = [&](int i) -> u64* { return ctl + (size_t)i * kCtlStride; };
     if (q.nvb[s] == 0) continue;
     if (q.wait[s]) {
-      // Wait for every earlier stage that has items (thread 0 polls).
+      // Wait for every earlier stage that has items (thread 0 polls).  A
+      // limit of 0 gives up before polling (the host's test of this path).
       if (tid == 0) {
+        if (q.time_limit == 0 && !err) {
+          __hip_atomic_store(word(kCtlErr), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          err = true;
+        }
         for (int t = confirmed + 1; t < s && !err; t++) {
           if (q.nvb[t] == 0 || persist_stage_size(a, q, t) == 0) continue;
           const u64 t0 = wall_clock64();
@@ -1864,7 +1869,7 @@ unsigned long long wall_clock_hz() {
 }
 
 hipError_t launch_persistent(const BatchArgs& a, bool clear, unsigned long long* ctl,
-                             unsigned long long* stamps, hipStream_t s) {
+                             unsigned long long* stamps, uint64_t wait_us, hipStream_t s) {
   if (!persistent_supported(a.tile_w, a.tile_h) || !ctl) return hipErrorInvalidValue;
   PersistPlan q;
   memset(&q, 0, sizeof(q));
@@ -1887,7 +1892,7 @@ hipError_t launch_persistent(const BatchArgs& a, bool clear, unsigned long long*
   }
   q.ctl = ctl;
   q.stamps = stamps;
-  q.time_limit = 2ull * wall_clock_hz();  // 2 s per wait
+  q.time_limit = wait_us * (wall_clock_hz() / 1000000ull);
   const int grid = std::max(persist_blocks_per_cu<16>() * std::max(a.num_cus, 1), kPQueues);
   PersistArgs pa;
   pa.a = a;

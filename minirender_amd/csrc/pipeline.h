@@ -149,7 +149,7 @@ struct PersistPlan {
   int32_t depth[kPStages];  // items per workgroup a stage is dealt out in (>= 1)
   int32_t wait[kPStages];   // 1: the stage waits until every earlier stage is complete
   unsigned long long* ctl;  // kCtlBytes of control words, zero before the launch
-  unsigned long long time_limit;  // wall-clock ticks one wait may take
+  unsigned long long time_limit;  // wall-clock ticks one wait may take (0: every wait gives up)
   // Profiling (null: off): per stage s, words kStampStride * (2 s) and
   // kStampStride * (2 s + 1) take the maximum over workgroups of ~(first
   // wall-clock tick of the stage) and of its last tick; zero before the launch.
@@ -159,8 +159,10 @@ constexpr int kStampStride = 32;
 constexpr size_t kStampBytes = (size_t)2 * kPStages * kStampStride * 8;
 // Launch the persistent kernel for a batch (16 x 16 tiles), on a grid of the
 // workgroups the GPU holds at once.  `clear`: run the clear stage.
+// wait_us: how long one wait may take, in microseconds (0: every wait gives
+// up at once, which tests of the failure path use).
 hipError_t launch_persistent(const BatchArgs& a, bool clear, unsigned long long* ctl,
-                             unsigned long long* stamps, hipStream_t s);
+                             unsigned long long* stamps, uint64_t wait_us, hipStream_t s);
 // Wall-clock ticks per second of the device's clock (wall_clock64).
 unsigned long long wall_clock_hz();
 bool persistent_supported(int tile_w, int tile_h);

@@ -18,6 +18,7 @@ from tests.parity import assert_parity, fuzz_frame
 pytestmark = pytest.mark.gpu
 
 KINDS = ("rgb8", "depth", "winner")
+MR_ERR_DEVICE = -8  # include/minirender_amd.h
 ALL = ("rgb8", "depth", "winner", "rgb", "normals", "points")
 
 
@@ -129,3 +130,32 @@ def test_persistent_needs_16_tiles(gpu_ctx):
             gpu_ctx.set_persistent(True)
     finally:
         gpu_ctx.set_tile(16, 16)
+
+
+def test_persistent_wait_gives_up(gpu_ctx):
+    """A wait limit of 0 makes every wait of the persistent kernel give up:
+    the launch drains (every workgroup exits), the batch fails with
+    MR_ERR_DEVICE, and the next batches on the same context are right again
+    (its control words and the counters of the stages that did not run are
+    cleared first)."""
+    fr = scenes.config_terrain(n=96, width=320, height=180, shapes=True)
+    other = fuzz_frame(11)
+    gpu_ctx.set_tile(16, 16)
+    ref = gpu_ctx.render(fr, ALL)
+    ref_other = gpu_ctx.render(other, ALL)
+    gpu_ctx.set_persistent(True)
+    try:
+        gpu_ctx.set_wait_limit(0)
+        for _ in range(2):
+            with pytest.raises(mra.MrError) as e:
+                gpu_ctx.render(fr, ALL)
+            assert e.value.code == MR_ERR_DEVICE
+        gpu_ctx.set_wait_limit(-1)
+        for f, r in ((fr, ref), (other, ref_other), (fr, ref)):
+            g = gpu_ctx.render(f, ALL)
+            assert gpu_ctx.stats().launches == 1
+            kinds = ALL if f.save_normals else tuple(k for k in ALL if k != "normals")
+            assert _same(g, r, kinds)
+    finally:
+        gpu_ctx.set_wait_limit(-1)
+        gpu_ctx.set_persistent(False)
