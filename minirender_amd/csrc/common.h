@@ -121,7 +121,7 @@ constexpr int kCkptSegs = 64;
 
 constexpr int kRegion = 128;       // item slots per geometry wavefront (64 x 2)
 constexpr int kShadeBlock = 256;   // pixels per deferred-shading workgroup
-constexpr int kShadeW = 64;        // shading block: kShadeW x kShadeH pixels
+constexpr int kShadeW = 16;        // shading block: kShadeW x kShadeH pixels
 constexpr int kShadeH = kShadeBlock / kShadeW;
 constexpr int kMaxLdsDraws = 256;  // draw-table entries cached in LDS
 
