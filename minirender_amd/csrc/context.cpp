@@ -480,7 +480,24 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
     memcpy(ht + p.layout.vg_draw, p.vg_draw.data(), p.vg_draw.size() * 4);
   }
   memcpy(ht + p.layout.hints, p.hints.data(), p.hints.size() * 8);
-  const BatchArgs a = make_args(c, s, p);
+  // The persistent kernel runs for the synchronous calls only: two of them
+  // in flight at once (the service's slots) would compete for residency.
+  const bool persistent = c->persistent && &s == &c->main && x.ctl.p &&
+                          persistent_supported(c->tile_w, c->tile_h);
+  BatchArgs a = make_args(c, s, p);
+  // The shading kernel writes the counters into the pinned read-back buffer
+  // (the persistent kernel sets its error word after shading: it keeps the
+  // copy).
+  if (!persistent) {
+    if (s.h_counters_of != x.h_counters.p) {
+      void* d = nullptr;
+      s.h_counters_dev = nullptr;
+      if (hipHostGetDevicePointer(&d, x.h_counters.p, 0) == hipSuccess) s.h_counters_dev = d;
+      else (void)hipGetLastError();
+      s.h_counters_of = x.h_counters.p;
+    }
+    a.host_counters = (unsigned long long*)s.h_counters_dev;
+  }
   s.last_geom_grid = p.total_tris ? geom_grid(a) : 0;
   s.last_bin_grid = bin_grid(a);
   // The clear kernel runs only when the counters it zeroes are not known to
@@ -496,17 +513,14 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
   const int64_t next_tiles = std::max<int64_t>(p.ntiles, same_bufs ? s.clean_tiles : 0);
   const int64_t next_scan = std::max<int64_t>(a.scan_blocks, same_bufs ? s.clean_scan : 0);
   s.clean_tiles = s.clean_scan = 0;  // until the launches below succeed
-  // The persistent kernel runs for the synchronous calls only: two of them
-  // in flight at once (the service's slots) would compete for residency.
-  const bool persistent = c->persistent && &s == &c->main && x.ctl.p &&
-                          persistent_supported(c->tile_w, c->tile_h);
   auto issue = [&]() -> int {
     HIP_TRY(hipMemcpyAsync(x.tables.p, x.h_tables.p, p.layout.bytes, hipMemcpyHostToDevice,
                            s.stream));
     int r = launch_stages(a, s, profile, clear, persistent, c->wait_us);
     if (r) return r;
-    HIP_TRY(hipMemcpyAsync(x.h_counters.p, a.counters, kNumCounters * 8, hipMemcpyDeviceToHost,
-                           s.stream));
+    if (!a.host_counters)
+      HIP_TRY(hipMemcpyAsync(x.h_counters.p, a.counters, kNumCounters * 8,
+                             hipMemcpyDeviceToHost, s.stream));
     return MR_OK;
   };
   // Graphs only on the slot's own stream (a caller's stream may be the null

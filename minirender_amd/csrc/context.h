@@ -97,6 +97,8 @@ struct Slot {
   hipStream_t own_stream = nullptr;
   hipEvent_t ev[kStages + 1] = {};  // stage boundaries (profiling)
   hipEvent_t done = nullptr;
+  void* h_counters_dev = nullptr;  // device address of sc.h_counters ...
+  void* h_counters_of = nullptr;   // ... for this host buffer
   uint32_t heavy_slots = 0;  // merge buffers of split tiles (grown on demand)
   size_t parts_want = 0;     // raster work items a batch needed (after an overflow)
   int32_t last_geom_grid = 0, last_bin_grid = 0;

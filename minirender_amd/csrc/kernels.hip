@@ -1390,6 +1390,11 @@ __device__ void reduce_stats(const BatchArgs& a, int tid) {
     }
     a.counters[kClipped] = sc;
     a.counters[kItems] = sr;
+    // Every counter is final here (the kernels that add to them are done):
+    // hand them to the host with the batch, no copy of its own.
+    if (a.host_counters)
+      for (int i = 0; i < kNumCounters; i++)
+        a.host_counters[i] = i == kClipped ? sc : (i == kItems ? sr : a.counters[i]);
   }
 }
 

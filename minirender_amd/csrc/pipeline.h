@@ -48,6 +48,10 @@ struct BatchArgs {
   uint32_t band_work;        // a tile with more walk work is rastered in row bands
   unsigned long long* scan_status;  // [scan_blocks] decoupled look-back words
   unsigned long long* counters;     // [kNumCounters]
+  // The counters' copy in pinned host memory (device address), written by
+  // the shading kernel's first workgroup once they are final; null: the
+  // host copies them back itself.
+  unsigned long long* host_counters;
   // Per-workgroup statistics (no shared counter: same-address atomics
   // serialize): [0, kStatSlots) clipped triangles per geometry workgroup,
   // [kStatSlots, 2 kStatSlots) binned triangles per bin workgroup.
