@@ -159,3 +159,28 @@ def test_persistent_wait_gives_up(gpu_ctx):
     finally:
         gpu_ctx.set_wait_limit(-1)
         gpu_ctx.set_persistent(False)
+
+
+def test_persistent_graph_replay(gpu_ctx):
+    """Persistent batches captured as HIP graphs: replays (and in-place
+    updates for other sizes) equal the eager separate launches, round after
+    round, with the control words cleared by each replayed launch."""
+    frames = [fuzz_frame(700 + i, 80 + 16 * i, 60 + 8 * i) for i in range(3)]
+    kinds = ("rgb8", "depth", "winner", "rgb")
+    gpu_ctx.set_tile(16, 16)
+    eager = [gpu_ctx.render(f, kinds) for f in frames]
+    ctx = mra.Context(0)
+    try:
+        ctx.set_tile(16, 16)
+        ctx.set_persistent(True)
+        ctx.set_graphs(True)
+        for rep in range(3):
+            for i, f in enumerate(frames):
+                g = ctx.render(f, kinds)
+                assert ctx.stats().launches == 1
+                assert _same(g, eager[i], kinds), (rep, i)
+        st = ctx.stats()
+    finally:
+        ctx.close()
+    assert st.graph_replays + st.graph_updates + st.graph_captures == 9
+    assert st.graph_replays + st.graph_updates >= 6
