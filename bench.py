@@ -625,7 +625,7 @@ def serve_study(args, torch, mra, reqs, graphs=True):
                      for k, v in modes.items()}}
     for where in ("device", "host"):
         res = {}
-        count = n if where == "device" else min(n, 10000)
+        count = n
         for name in modes:
             res[name] = trace(name, arrivals(args.serve_rate, count), count, where)
             log(f"serving {where} {name}: {res[name]}")
