@@ -199,8 +199,20 @@ __device__ inline JobDesc uniform_job(const BatchArgs& a, int j) {
   return out;
 }
 
+// Global-memory view of a device pointer read from a descriptor.  The
+// compiler cannot tell where such a pointer points and accesses it with
+// flat instructions, which also count against the scalar-load counter: every
+// wait for a descriptor field then waits for them too.  Through this view
+// they are global loads and stores (vector memory only), with a scalar base
+// and 32-bit offsets where the index allows.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 __device__ inline f3 load3(const float* p, int i) {
-  return f3{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+  const auto* q = G(p);
+  return f3{q[3 * i], q[3 * i + 1], q[3 * i + 2]};
 }
 
 // Projected-vertex record of vertex vi of draw D (written by k_vertex).
@@ -216,7 +228,7 @@ __device__ inline void load_vrec(const BatchArgs& a, const DrawDesc& D, int vi, 
 // 364-366).
 __device__ inline void corners(const BatchArgs& a, const DrawDesc& D, int t, f3 e[3],
                                ScreenVtx v[3]) {
-  const int* ip = D.idx + 3 * t;
+  const auto* ip = G(D.idx) + 3 * t;
   const int i0 = ip[0], i1 = ip[1], i2 = ip[2];
   load_vrec(a, D, i0, e[0], v[0]);
   load_vrec(a, D, i1, e[1], v[1]);
@@ -1100,7 +1112,7 @@ __device__ __forceinline__ void stage_tile(const BatchArgs& a, int64_t b, bool r
         const int c = x0 + (p % TW), row = y0 + (p / TW), lp = (p / TW) * KS + (p % TW);
         if (c < J.W && row < J.H) {
           const size_t pix = (size_t)row * J.W + c;
-          J.win[pix] = keys[lp] == ~0ull ? kNoWinner : (uint32_t)keys[lp];
+          G(J.win)[pix] = keys[lp] == ~0ull ? kNoWinner : (uint32_t)keys[lp];
         }
       }
     } else {
@@ -1132,7 +1144,7 @@ __device__ __forceinline__ void stage_tile(const BatchArgs& a, int64_t b, bool r
             const u64 k = __hip_atomic_load(&gk[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (c < J.W && row < J.H) {
               const size_t pix = (size_t)row * J.W + c;
-              J.win[pix] = k == ~0ull ? kNoWinner : (uint32_t)k;
+              G(J.win)[pix] = k == ~0ull ? kNoWinner : (uint32_t)k;
             }
           }
         }
@@ -1149,9 +1161,10 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
 // Range image of the pixel from its depth (Renderer.cpp:388-415).
 __device__ inline void store_point(const JobDesc& J, size_t pix, int c, int row, float depth) {
   const f3 q = range_point(J.P, range_far_depth(J.P), (float)J.W, (float)J.H, row, c, depth);
-  J.points[pix * 3 + 0] = q.x;
-  J.points[pix * 3 + 1] = q.y;
-  J.points[pix * 3 + 2] = q.z;
+  auto* o = G(J.points) + pix * 3;
+  o[0] = q.x;
+  o[1] = q.y;
+  o[2] = q.z;
 }
 
 // Reciprocal length and normalisation with the hardware reciprocal square
@@ -1167,8 +1180,8 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   const int tri = (int)(g - D.tri_begin);
   // All index loads first, then all attribute loads: two dependent levels
   // of memory access instead of four.
-  const int* ip = D.idx + 3 * tri;
-  const int* in = D.nidx + 3 * tri;
+  const auto* ip = G(D.idx) + 3 * tri;
+  const auto* in = G(D.nidx) + 3 * tri;
   // The pixel's checkpoint needs only the triangle's checkpoint base, its
   // row and tile column: it is loaded with the vertex records.
   const int sw = __ffs(a.tile_w) - 1;
@@ -1189,7 +1202,7 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
     t1 = i1;
     t2 = i2;
   } else if (D.tidx) {
-    const int* it = D.tidx + 3 * tri;
+    const auto* it = G(D.tidx) + 3 * tri;
     t0 = it[0];
     t1 = it[1];
     t2 = it[2];
@@ -1202,9 +1215,10 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   const f3 N0 = load3(D.nrm, n0), N1 = load3(D.nrm, n1), N2 = load3(D.nrm, n2);
   f2 U0 = f2{0.0f, 0.0f}, U1 = U0, U2 = U0;  // Renderer.cpp:371-379
   if (D.tidx) {
-    U0 = f2{D.uv[2 * t0], D.uv[2 * t0 + 1]};
-    U1 = f2{D.uv[2 * t1], D.uv[2 * t1 + 1]};
-    U2 = f2{D.uv[2 * t2], D.uv[2 * t2 + 1]};
+    const auto* uv = G(D.uv);
+    U0 = f2{uv[2 * t0], uv[2 * t0 + 1]};
+    U1 = f2{uv[2 * t1], uv[2 * t1 + 1]};
+    U2 = f2{uv[2 * t2], uv[2 * t2 + 1]};
   }
   // (The entry of the triangle's first tile column is read too but not used:
   // its rows start at the bbox.)
@@ -1216,7 +1230,10 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   v1.p = ec[1];
   v2.p = ec[2];
   TriSetup s;
-  tri_setup_corners(J, ec, vc, sub, D.no_clip != 0, s);
+  if (!D.no_clip && (ec[0].z > J.znear || ec[1].z > J.znear || ec[2].z > J.znear))
+    tri_setup_corners(J, ec, vc, sub, false, s);
+  else
+    setup_edges((float)J.W, vc[0], vc[1], vc[2], s);
   // Edge values at the pixel, from the same serial walk along the row as the
   // raster's: from the row pre-pass's checkpoint where the row enters the
   // pixel's tile column, or from the bbox's left column.
@@ -1233,17 +1250,29 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   float k[3];
   const float z = depth_of(e1, e2, s.z0, s.z1, s.z2, J.persp != 0, k);
   if (!J.rgb8 && !J.rgb && !(J.normals && J.save_normals)) {
-    if (J.depth) J.depth[pix] = z;
-    if (J.winner) J.winner[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
+    if (J.depth) G(J.depth)[pix] = z;
+    if (J.winner) G(J.winner)[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
     if (J.points) store_point(J, pix, c, row, z);
     return;
   }
 
   // Normals and uvs of the triangle, re-clipped for a clipped sub-triangle
-  // (Renderer.cpp:131-161).
-  v0.n = xform(D.nm, N0);
-  v1.n = xform(D.nm, N1);
-  v2.n = xform(D.nm, N2);
+  // (Renderer.cpp:131-161).  The reference moves each vertex normal into eye
+  // space and interpolates (:347-348, 285-288); the saved normals image keeps
+  // that order (bit-exact).  For colour alone the object-space normals are
+  // interpolated and moved once: the same affine map of the same weights
+  // (they sum to 1, and clipping interpolates linearly), equal up to
+  // rounding, with two matrix products fewer per pixel.
+  const bool save_n = J.normals && J.save_normals;
+  if (save_n) {
+    v0.n = xform(D.nm, N0);
+    v1.n = xform(D.nm, N1);
+    v2.n = xform(D.nm, N2);
+  } else {
+    v0.n = N0;
+    v1.n = N1;
+    v2.n = N2;
+  }
   v0.uv = U0;
   v1.uv = U1;
   v2.uv = U2;
@@ -1263,15 +1292,15 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   }
   f3 value = f3{D.emissive[0], D.emissive[1], D.emissive[2]};
   if (J.lighting) {  // :279-304, FAST_LIGHT
-    // Colour only (the contract is 1 LSB of RGB8): hardware reciprocal,
-    // reciprocal square root and exp2/log2 replace the correctly rounded
-    // divide, sqrt and powf; the error is a few ulp of fp32.
+    // Colour only (the contract is 1 LSB of RGB8): hardware reciprocal
+    // square root and exp2/log2 (v_rsq, v_log, v_exp) replace the correctly
+    // rounded sqrt, divide and powf; the error is a few ulp of fp32.
     const f3 position = add3(add3(scale3(k[0], v0.p), scale3(k[1], v1.p)),
                              scale3(k[2], v2.p));
     const f3 L = f3{J.light[0], J.light[1], J.light[2]};
     const f3 lightdir = J.light_is_point ? fast_norm3(sub3(L, position)) : L;
-    const f3 normal = add3(add3(scale3(k[0], v0.n), scale3(k[1], v1.n)),
-                           scale3(k[2], v2.n));
+    f3 normal = add3(add3(scale3(k[0], v0.n), scale3(k[1], v1.n)), scale3(k[2], v2.n));
+    if (!save_n) normal = xform(D.nm, normal);
     const float inl = fast_rlen3(normal);
     const float diff = amax(0.0f, dot3(normal, lightdir)) * inl + J.ambient;
     value = add3(value, scale3(diff, color));
@@ -1279,52 +1308,61 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
       const f3 view = fast_norm3(position);
       const f3 hv = sub3(lightdir, view);
       const float cosa = amax(dot3(hv, normal), 0.0f) * (fast_rlen3(hv) * inl);
-      const float sp = __powf(cosa, D.shininess);
+      // powf(cosa, shininess) as exp2(shininess * log2(cosa)): cosa = 0
+      // gives 0 (or inf for a negative exponent), as powf does.
+      const float sp = __builtin_amdgcn_exp2f(D.shininess * __builtin_amdgcn_logf(cosa));
       value = add3(value, scale3(sp, f3{D.specular[0], D.specular[1], D.specular[2]}));
     }
-    if (J.normals && J.save_normals) {
-      J.normals[pix * 3 + 0] = normal.x;
-      J.normals[pix * 3 + 1] = normal.y;
-      J.normals[pix * 3 + 2] = normal.z;
+    if (save_n) {
+      auto* o = G(J.normals) + pix * 3;
+      o[0] = normal.x;
+      o[1] = normal.y;
+      o[2] = normal.z;
     }
-  } else if (J.normals && J.save_normals) {
-    J.normals[pix * 3 + 0] = 0.0f;
-    J.normals[pix * 3 + 1] = 0.0f;
-    J.normals[pix * 3 + 2] = 1.0f;
+  } else if (save_n) {
+    auto* o = G(J.normals) + pix * 3;
+    o[0] = 0.0f;
+    o[1] = 0.0f;
+    o[2] = 1.0f;
   }
   if (J.rgb8) {
-    J.rgb8[pix * 3 + 0] = quantize(value.x);
-    J.rgb8[pix * 3 + 1] = quantize(value.y);
-    J.rgb8[pix * 3 + 2] = quantize(value.z);
+    auto* o = G(J.rgb8) + pix * 3;
+    o[0] = quantize(value.x);
+    o[1] = quantize(value.y);
+    o[2] = quantize(value.z);
   }
   if (J.rgb) {
-    J.rgb[pix * 3 + 0] = value.x;
-    J.rgb[pix * 3 + 1] = value.y;
-    J.rgb[pix * 3 + 2] = value.z;
+    auto* o = G(J.rgb) + pix * 3;
+    o[0] = value.x;
+    o[1] = value.y;
+    o[2] = value.z;
   }
-  if (J.depth) J.depth[pix] = z;
-  if (J.winner) J.winner[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
+  if (J.depth) G(J.depth)[pix] = z;
+  if (J.winner) G(J.winner)[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
   if (J.points) store_point(J, pix, c, row, z);
 }
 
 // A pixel nothing was drawn on: clear(), Renderer.cpp:113-119.
 __device__ inline void shade_background(const JobDesc& J, size_t pix, int c, int row) {
   if (J.rgb8) {
-    J.rgb8[pix * 3 + 0] = quantize(J.bg[0]);
-    J.rgb8[pix * 3 + 1] = quantize(J.bg[1]);
-    J.rgb8[pix * 3 + 2] = quantize(J.bg[2]);
+    auto* o = G(J.rgb8) + pix * 3;
+    o[0] = quantize(J.bg[0]);
+    o[1] = quantize(J.bg[1]);
+    o[2] = quantize(J.bg[2]);
   }
-  if (J.depth) J.depth[pix] = kClearDepth;
-  if (J.winner) J.winner[pix] = kNoWinner;
+  if (J.depth) G(J.depth)[pix] = kClearDepth;
+  if (J.winner) G(J.winner)[pix] = kNoWinner;
   if (J.rgb) {
-    J.rgb[pix * 3 + 0] = J.bg[0];
-    J.rgb[pix * 3 + 1] = J.bg[1];
-    J.rgb[pix * 3 + 2] = J.bg[2];
+    auto* o = G(J.rgb) + pix * 3;
+    o[0] = J.bg[0];
+    o[1] = J.bg[1];
+    o[2] = J.bg[2];
   }
   if (J.normals && J.save_normals) {
-    J.normals[pix * 3 + 0] = 0.0f;
-    J.normals[pix * 3 + 1] = 0.0f;
-    J.normals[pix * 3 + 2] = 1.0f;
+    auto* o = G(J.normals) + pix * 3;
+    o[0] = 0.0f;
+    o[1] = 0.0f;
+    o[2] = 1.0f;
   }
   if (J.points) store_point(J, pix, c, row, kClearDepth);
 }
@@ -1428,7 +1466,7 @@ __device__ __forceinline__ void stage_shade(const BatchArgs& a, int vb, bool rem
   if (inside) {
     const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
     const uint32_t flag = a.tile_heavy[J.tile_base + (row >> sh) * J.tiles_x + (c >> sw)];
-    if (flag != kEmptyTile) id = J.win[pix];
+    if (flag != kEmptyTile) id = G(J.win)[pix];
   }
   // A job of one draw needs no draw table (ncached = -1).
   const bool one_draw = J.draw_end - J.draw_begin == 1;

@@ -2,7 +2,7 @@
 
 Bar: winner (draw-order index of the visible triangle) and depth bits equal,
 RGB8 colour within 1 LSB per channel, unquantized colour within 1e-5 relative
-(powf of libm vs the device library), normals and range image bit-equal.
+(hardware exp2/log2 and rsqrt in the colour), normals and range image bit-equal.
 """
 from __future__ import annotations
 
