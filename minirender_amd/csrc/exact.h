@@ -392,12 +392,13 @@ MR_HD inline bool setup_screen(float w, float h, const ScreenVtx& v0, const Scre
 }
 
 // What shading needs of setup_screen for a triangle that passed it (a
-// winner): edge normals, vertices 0 and 2, depths and the bbox's left
-// column, by the same arithmetic.
+// winner): edge normals, vertices 0 and 2, depths and the bbox's columns,
+// by the same arithmetic.
 MR_HD inline void setup_edges(float w, const ScreenVtx& v0, const ScreenVtx& v1,
                               const ScreenVtx& v2, TriSetup& r) {
   const float p0x = v0.px, p0y = v0.py, p1x = v1.px, p1y = v1.py, p2x = v2.px, p2y = v2.py;
   const float minx = amin(amin(amin(1e30f, p0x), p1x), p2x);
+  const float maxx = amax(amax(amax(-1e30f, p0x), p1x), p2x);
   const float ar = (p0x - p1x) * (p2y - p1y) - (p0y - p1y) * (p2x - p1x);
   const float i2a = -1.0f / ar;
   r.n1x = -(p0y - p2y) * i2a;
@@ -406,6 +407,7 @@ MR_HD inline void setup_edges(float w, const ScreenVtx& v0, const ScreenVtx& v1,
   r.n2y = (p1x - p0x) * i2a;
   r.p0x = p0x; r.p0y = p0y; r.p2x = p2x; r.p2y = p2y;
   r.bx0 = (int)floorf(aclamp(minx, 0.0f, w - 1));
+  r.bx1 = last_center(aclamp(maxx, 0.0f, w - 1));
   r.z0 = v0.zv;
   r.z1 = v1.zv;
   r.z2 = v2.zv;

@@ -757,9 +757,10 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) { stage_scan(a
 // the high 24.  Every such row holds at least two entries, so a segment's
 // row tasks fit in half its entry count.  The base is stored pre-offset by
 // the item's first row and column, so that a reader needs only the row and
-// the tile column.  Items without checkpoints (narrower, or no room left)
-// get {0, 0}: the raster and shading walk their rows from the bbox's left
-// column (fewer than kCkptMinSpan steps beyond a tile).
+// the tile column.  Items narrower than kCkptMinSpan pixels get no entry
+// (their readers test the width first); wide ones that found no room get
+// {0, 0}.  Either way the raster and shading walk such rows from the bbox's
+// left column (fewer than kCkptMinSpan steps beyond a tile for the narrow).
 constexpr int kCkptMinSpan = 64;
 constexpr u64 kCkptEntryMask = (1ull << 40) - 1;
 
@@ -770,10 +771,7 @@ __device__ inline void ckpt_alloc(const BatchArgs& a, bool act, const BinItem& i
   const int ntx = act ? (it.bx1 >> sw) - btx + 1 : 0;
   const int rows = it.by1 - it.by0 + 1;
   const bool want = ntx > 1 && it.bx1 - it.bx0 + 1 >= kCkptMinSpan;
-  if (__ballot(want) == 0ull) {  // the common case: no wide triangle here
-    if (act) a.ckpt_base[it.id] = make_int2(0, 0);
-    return;
-  }
+  if (__ballot(want) == 0ull) return;  // the common case: no wide triangle here
   const u64 need = want ? (u64)ntx * (u64)rows : 0ull;
   const uint32_t nrow = want ? (uint32_t)rows : 0u;
   u64 P = need;  // inclusive prefixes over the wave
@@ -789,7 +787,7 @@ __device__ inline void ckpt_alloc(const BatchArgs& a, bool act, const BinItem& i
   }
   const u64 total = __shfl(P, 63);
   if (total == 0ull) {
-    if (act) a.ckpt_base[it.id] = make_int2(0, 0);
+    if (want) a.ckpt_base[it.id] = make_int2(0, 0);
     return;
   }
   const uint32_t rtotal = __shfl(R, 63);
@@ -801,7 +799,7 @@ __device__ inline void ckpt_alloc(const BatchArgs& a, bool act, const BinItem& i
   old = __shfl(old, 63);
   const u64 base = old & kCkptEntryMask, rbase = old >> 40;
   bool ok = false;
-  if (act) {
+  if (want) {
     const u64 first = base + P - need;  // within the segment
     const int64_t off = (int64_t)(seg * seg_cap + first) - (int64_t)it.by0 * ntx - btx;
     ok = need && first + need <= seg_cap && off >= INT32_MIN && off <= INT32_MAX;
@@ -1182,11 +1180,8 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   // of memory access instead of four.
   const auto* ip = G(D.idx) + 3 * tri;
   const auto* in = G(D.nidx) + 3 * tri;
-  // The pixel's checkpoint needs only the triangle's checkpoint base, its
-  // row and tile column: it is loaded with the vertex records.
   const int sw = __ffs(a.tile_w) - 1;
   const int ctx = c >> sw;
-  const int2 cb = a.ckpt_base[id];
   const int i0 = ip[0], i1 = ip[1], i2 = ip[2];
   // Normal and texcoord indices that share the position index buffer are
   // not loaded again.
@@ -1220,10 +1215,6 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
     U1 = f2{uv[2 * t1], uv[2 * t1 + 1]};
     U2 = f2{uv[2 * t2], uv[2 * t2 + 1]};
   }
-  // (The entry of the triangle's first tile column is read too but not used:
-  // its rows start at the bbox.)
-  float2 st = make_float2(0.0f, 0.0f);
-  if (cb.y) st = a.ckpt[(int64_t)cb.x + (int64_t)row * cb.y + ctx];
   // Eye-space vertices as paintMesh computes them (Renderer.cpp:344-349).
   Vtx v0, v1, v2;
   v0.p = ec[0];
@@ -1236,13 +1227,22 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
     setup_edges((float)J.W, vc[0], vc[1], vc[2], s);
   // Edge values at the pixel, from the same serial walk along the row as the
   // raster's: from the row pre-pass's checkpoint where the row enters the
-  // pixel's tile column, or from the bbox's left column.
-  float e1, e2;
-  if (cb.y && ctx > (s.bx0 >> sw)) {
-    e1 = st.x;
-    e2 = st.y;
-    advance2(e1, s.n1x, e2, s.n2x, c - (ctx << sw));
-  } else {
+  // pixel's tile column, or from the bbox's left column.  Only triangles at
+  // least kCkptMinSpan pixels wide can have checkpoints (the raster's test):
+  // the others read no checkpoint base.
+  float e1 = 0.0f, e2 = 0.0f;
+  bool from_ckpt = false;
+  if (ctx > (s.bx0 >> sw) && s.bx1 - s.bx0 + 1 >= kCkptMinSpan) {
+    const int2 cb = a.ckpt_base[id];
+    if (cb.y) {
+      const float2 st = a.ckpt[(int64_t)cb.x + (int64_t)row * cb.y + ctx];
+      e1 = st.x;
+      e2 = st.y;
+      advance2(e1, s.n1x, e2, s.n2x, c - (ctx << sw));
+      from_ckpt = true;
+    }
+  }
+  if (!from_ckpt) {
     edge_start(s.n1x, s.n1y, s.n2x, s.n2y, s.p0x, s.p0y, s.p2x, s.p2y,
                (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
     advance2(e1, s.n1x, e2, s.n2x, c - s.bx0);
