@@ -119,6 +119,28 @@ __device__ inline int search_last(const T* begin, int lo, int hi, int64_t x) {
   return lo;
 }
 
+// search_last by a whole wavefront (every lane active, x uniform): the 64
+// lanes probe 64 evenly spaced entries per step, so n entries take
+// ceil(log64 n) dependent loads instead of log2 n (the first workgroup
+// step of the vertex and geometry kernels waits on this search).
+// begin[0] <= x.
+__device__ inline int wave_search_last(const int64_t* begin, int n, int64_t x) {
+  const int lane = threadIdx.x & 63;
+  int lo = 0, span = n;  // the answer lies in [lo, lo + span)
+  while (span > 1) {
+    const int step = (span + 63) >> 6;
+    const int idx = lo + lane * step;
+    const bool le = idx < lo + span && begin[idx] <= x;
+    const u64 m = __ballot(le);  // lane 0 (idx = lo) is always set
+    const int k = 63 - __builtin_clzll(m);
+    const int nlo = lo + k * step;
+    span = min(step, lo + span - nlo);
+    lo = __builtin_amdgcn_readfirstlane(nlo);
+    span = __builtin_amdgcn_readfirstlane(span);
+  }
+  return lo;
+}
+
 // Job of tile (shading block) x from the job hints: the job of x's span of
 // kHintSpan, stepped past job boundaries inside the span (at most one unless
 // jobs are smaller than the span).  One scalar load where a binary search
@@ -351,8 +373,7 @@ __device__ __forceinline__ void stage_vertex(const BatchArgs& a, int vb, int nvb
   const int64_t r_begin = (int64_t)vb * chunk;
   const int64_t r_end = min(r_begin + chunk, a.total_verts);
   const int lane = threadIdx.x & 63;
-  int gw = search_last(a.vg_begin, 0, a.nvgroups - 1,
-                       (int64_t)__builtin_amdgcn_readfirstlane((int)min(r_begin, a.total_verts)));
+  int gw = wave_search_last(a.vg_begin, a.nvgroups, min(r_begin, a.total_verts));
   gw = __builtin_amdgcn_readfirstlane(gw);
   for (int64_t base = r_begin; base < r_end; base += kVertexStep) {
     const int64_t wfirst = base + kVertsPerLane * (threadIdx.x & ~63);
@@ -515,8 +536,7 @@ __device__ __forceinline__ void stage_geom(const BatchArgs& a, int vb, int nvb) 
                          ((int64_t)nvb * 256)) * 256;
   const int64_t r_begin = (int64_t)vb * chunk;
   const int64_t r_end = min(r_begin + chunk, a.total_tris);
-  int dw = search_last(a.draw_begin, 0, a.ndraws - 1,
-                       (int64_t)__builtin_amdgcn_readfirstlane((int)min(r_begin, a.total_tris)));
+  int dw = wave_search_last(a.draw_begin, a.ndraws, min(r_begin, a.total_tris));
   dw = __builtin_amdgcn_readfirstlane(dw);
   for (int64_t base = r_begin; base < r_end; base += 256) {
     const int64_t wfirst = base + (threadIdx.x & ~63);
