@@ -668,7 +668,12 @@ __device__ __forceinline__ void stage_scan(const BatchArgs& a) {
   if (tid == 0) {
     const u64 excl = lookback(a.scan_status, blk, btot, a.counters);
     sh_excl = excl;
-    if (blk == a.scan_blocks - 1) a.counters[kBinEntries] = excl + btot;
+    if (blk == a.scan_blocks - 1) {
+      a.counters[kBinEntries] = excl + btot;
+      // More pairs than the bin list holds (32-bit offsets and cursors could
+      // then wrap into it): flagged here, whatever the binner's checks see.
+      if (excl + btot > a.bins_cap) atomicOr(&a.counters[kBinOverflow], 1ull);
+    }
   }
   // Split (heavy) tiles: the workgroup reserves its merge slots and part
   // list entries with one atomic each (same-address atomics serialise); the
