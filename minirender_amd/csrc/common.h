@@ -70,8 +70,11 @@ struct JobDesc {
 };
 
 // Projected-vertex record written once per vertex by the vertex kernel:
-// eye-space position (Renderer.cpp:344-345) and its projection.
-constexpr int kVrecFloats = 6;  // ex, ey, ez, px, py, zv
+// eye-space position (Renderer.cpp:344-345) and its projection, in two
+// sections of one buffer: {px, py, zv, ez} (16 B, all that triangle setup
+// reads for an unclipped triangle) for every vertex, then {ex, ey} (8 B,
+// read by the near-plane clip and by shading).
+constexpr int kVrecFloats = 6;  // floats per vertex over both sections
 
 // A surviving (sub-)triangle as the binner sees it: its id (2 * flat
 // triangle index + clip sub-index), its draw and clamped pixel bbox.

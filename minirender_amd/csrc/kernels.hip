@@ -237,24 +237,43 @@ __device__ inline f3 load3(const float* p, int i) {
   return f3{q[3 * i], q[3 * i + 1], q[3 * i + 2]};
 }
 
-// Projected-vertex record of vertex vi of draw D (written by k_vertex).
+// Projected-vertex record of vertex vi of draw D (written by k_vertex):
+// eye position and projection, both sections.
 __device__ inline void load_vrec(const BatchArgs& a, const DrawDesc& D, int vi, f3& e,
                                  ScreenVtx& v) {
-  const float2* r = reinterpret_cast<const float2*>(a.vrec + (D.vbase + vi) * kVrecFloats);
-  const float2 r0 = r[0], r1 = r[1], r2 = r[2];
-  e = f3{r0.x, r0.y, r1.x};
-  v = ScreenVtx{r1.y, r2.x, r2.y};
+  const int64_t k = D.vbase + vi;
+  const float4 r = a.vscr[k];
+  const float2 q = a.veye[k];
+  e = f3{q.x, q.y, r.w};
+  v = ScreenVtx{r.x, r.y, r.z};
 }
 
-// Eye-space corners and projections of triangle t (Renderer.cpp:344-345,
-// 364-366).
-__device__ inline void corners(const BatchArgs& a, const DrawDesc& D, int t, f3 e[3],
-                               ScreenVtx v[3]) {
+// Corner indices of triangle t, their projections and eye depths
+// (Renderer.cpp:344-345, 364-366): one 16-byte load per corner.  The eye
+// x, y are left 0: corner_eyes loads them where the near-plane clip needs
+// them.
+__device__ inline void corners(const BatchArgs& a, const DrawDesc& D, int t, int vi[3],
+                               f3 e[3], ScreenVtx v[3]) {
   const auto* ip = G(D.idx) + 3 * t;
-  const int i0 = ip[0], i1 = ip[1], i2 = ip[2];
-  load_vrec(a, D, i0, e[0], v[0]);
-  load_vrec(a, D, i1, e[1], v[1]);
-  load_vrec(a, D, i2, e[2], v[2]);
+  vi[0] = ip[0];
+  vi[1] = ip[1];
+  vi[2] = ip[2];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float4 r = a.vscr[D.vbase + vi[k]];
+    v[k] = ScreenVtx{r.x, r.y, r.z};
+    e[k] = f3{0.0f, 0.0f, r.w};
+  }
+}
+
+__device__ inline void corner_eyes(const BatchArgs& a, const DrawDesc& D, const int vi[3],
+                                   f3 e[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float2 q = a.veye[D.vbase + vi[k]];
+    e[k].x = q.x;
+    e[k].y = q.y;
+  }
 }
 
 // Near test, clip and setup of triangle t (Renderer.cpp:169-236).  Returns
@@ -264,9 +283,10 @@ __device__ inline void corners(const BatchArgs& a, const DrawDesc& D, int t, f3 
 __device__ inline int tri_setup(const BatchArgs& a, const DrawDesc& D, const JobDesc& J, int t,
                                 TriSetup& s0, int& sub0, TriSetup& s1,
                                 int& sub1, bool& clipped) {
+  int vi[3];
   f3 e[3];
   ScreenVtx v[3];
-  corners(a, D, t, e, v);
+  corners(a, D, t, vi, e, v);
   const float zn = J.znear;
   const bool persp = J.persp != 0;
   const float w = (float)J.W, h = (float)J.H;
@@ -275,6 +295,7 @@ __device__ inline int tri_setup(const BatchArgs& a, const DrawDesc& D, const Job
   if (!D.no_clip && (b0 || b1 || b2)) {
     if (b0 && b1 && b2) return 0;
     clipped = true;
+    corner_eyes(a, D, vi, e);
     f3 a0, a1, a2, c0, c1, c2;
     const int ns = clip_positions(zn, e[0], e[1], e[2], a0, a1, a2, c0, c1, c2);
     int n = 0;
@@ -340,9 +361,12 @@ __device__ inline void tri_setup_corners(const JobDesc& J, const f3 e[3], const 
 // Setup of sub-triangle `sub` of a binned triangle (which survived setup).
 __device__ inline void tri_setup_sub(const BatchArgs& a, const DrawDesc& D, const JobDesc& J,
                                      int t, int sub, TriSetup& s) {
+  int vi[3];
   f3 e[3];
   ScreenVtx v[3];
-  corners(a, D, t, e, v);
+  corners(a, D, t, vi, e, v);
+  if (!D.no_clip && (e[0].z > J.znear || e[1].z > J.znear || e[2].z > J.znear))
+    corner_eyes(a, D, vi, e);
   tri_setup_corners(J, e, v, sub, D.no_clip != 0, s);
 }
 
@@ -359,10 +383,8 @@ __device__ inline void vertex_record(const BatchArgs& a, const DrawDesc& D, cons
                                      int64_t i, f3 p) {
   const f3 e = xform(D.mv, p);
   const ScreenVtx v = project_vertex(J.P, J.persp != 0, (float)J.W, (float)J.H, e);
-  float2* r = reinterpret_cast<float2*>(a.vrec + i * kVrecFloats);
-  r[0] = make_float2(e.x, e.y);
-  r[1] = make_float2(e.z, v.px);
-  r[2] = make_float2(v.py, v.zv);
+  a.vscr[i] = make_float4(v.px, v.py, v.zv, e.z);
+  a.veye[i] = make_float2(e.x, e.y);
 }
 
 __device__ __forceinline__ void stage_vertex(const BatchArgs& a, int vb, int nvb) {
