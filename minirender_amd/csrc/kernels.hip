@@ -237,17 +237,6 @@ __device__ inline f3 load3(const float* p, int i) {
   return f3{q[3 * i], q[3 * i + 1], q[3 * i + 2]};
 }
 
-// Projected-vertex record of vertex vi of draw D (written by k_vertex):
-// eye position and projection, both sections.
-__device__ inline void load_vrec(const BatchArgs& a, const DrawDesc& D, int vi, f3& e,
-                                 ScreenVtx& v) {
-  const int64_t k = D.vbase + vi;
-  const float4 r = a.vscr[k];
-  const float2 q = a.veye[k];
-  e = f3{q.x, q.y, r.w};
-  v = ScreenVtx{r.x, r.y, r.z};
-}
-
 // Corner indices of triangle t, their projections and eye depths
 // (Renderer.cpp:344-345, 364-366): one 16-byte load per corner.  The eye
 // x, y are left 0: corner_eyes loads them where the near-plane clip needs
@@ -1217,6 +1206,22 @@ __device__ inline void store_point(const JobDesc& J, size_t pix, int c, int row,
 __device__ inline float fast_rlen3(f3 a) { return __builtin_amdgcn_rsqf(dot3(a, a)); }
 __device__ inline f3 fast_norm3(f3 a) { return scale3(fast_rlen3(a), a); }
 
+// Eye-space point on the ray through the centre of pixel (c, row) at eye
+// depth ze: the projection (project_vertex) solved for eye x, y, with the
+// hardware reciprocal (colour only).
+__device__ inline f3 unproject_pixel(const JobDesc& J, int c, int row, float ze) {
+  const float* P = J.P;
+  const float xn = ((float)c + 0.5f) * (2.0f * __builtin_amdgcn_rcpf((float)J.W)) - 1.0f;
+  const float yn = 1.0f - ((float)row + 0.5f) * (2.0f * __builtin_amdgcn_rcpf((float)J.H));
+  const float a11 = P[0] - xn * P[12], a12 = P[1] - xn * P[13];
+  const float a21 = P[4] - yn * P[12], a22 = P[5] - yn * P[13];
+  const float w = P[14] * ze + P[15];
+  const float b1 = xn * w - (P[2] * ze + P[3]);
+  const float b2 = yn * w - (P[6] * ze + P[7]);
+  const float idet = __builtin_amdgcn_rcpf(a11 * a22 - a12 * a21);
+  return f3{(b1 * a22 - b2 * a12) * idet, (a11 * b2 - a21 * b1) * idet, ze};
+}
+
 // Shading of a covered pixel whose winner `id` belongs to draw D.
 __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, const DrawDesc& D,
                                           uint32_t id, int c, int row, size_t pix) {
@@ -1249,11 +1254,18 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
     t1 = it[1];
     t2 = it[2];
   }
+  // The corners' projections and eye depths; their eye x, y only for a
+  // triangle re-clipped below or of a draw without the near test (whose
+  // corners may lie behind the eye).
+  const int vi[3] = {i0, i1, i2};
   f3 ec[3];
   ScreenVtx vc[3];
-  load_vrec(a, D, i0, ec[0], vc[0]);
-  load_vrec(a, D, i1, ec[1], vc[1]);
-  load_vrec(a, D, i2, ec[2], vc[2]);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float4 r = a.vscr[D.vbase + vi[k]];
+    vc[k] = ScreenVtx{r.x, r.y, r.z};
+    ec[k] = f3{0.0f, 0.0f, r.w};
+  }
   const f3 N0 = load3(D.nrm, n0), N1 = load3(D.nrm, n1), N2 = load3(D.nrm, n2);
   f2 U0 = f2{0.0f, 0.0f}, U1 = U0, U2 = U0;  // Renderer.cpp:371-379
   if (D.tidx) {
@@ -1262,13 +1274,16 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
     U1 = f2{uv[2 * t1], uv[2 * t1 + 1]};
     U2 = f2{uv[2 * t2], uv[2 * t2 + 1]};
   }
+  const bool clip = !D.no_clip && (ec[0].z > J.znear || ec[1].z > J.znear || ec[2].z > J.znear);
+  const bool eyes = clip || D.no_clip;
+  if (eyes) corner_eyes(a, D, vi, ec);
   // Eye-space vertices as paintMesh computes them (Renderer.cpp:344-349).
   Vtx v0, v1, v2;
   v0.p = ec[0];
   v1.p = ec[1];
   v2.p = ec[2];
   TriSetup s;
-  if (!D.no_clip && (ec[0].z > J.znear || ec[1].z > J.znear || ec[2].z > J.znear))
+  if (clip)
     tri_setup_corners(J, ec, vc, sub, false, s);
   else
     setup_edges((float)J.W, vc[0], vc[1], vc[2], s);
@@ -1323,7 +1338,7 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   v0.uv = U0;
   v1.uv = U1;
   v2.uv = U2;
-  if (!D.no_clip && (v0.p.z > J.znear || v1.p.z > J.znear || v2.p.z > J.znear)) {
+  if (clip) {
     Vtx a0, a1, a2, b0, b1, b2;
     clip_triangle(J.znear, v0, v1, v2, a0, a1, a2, b0, b1, b2);
     if (sub) { v0 = b0; v1 = b1; v2 = b2; }
@@ -1342,8 +1357,14 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
     // Colour only (the contract is 1 LSB of RGB8): hardware reciprocal
     // square root and exp2/log2 (v_rsq, v_log, v_exp) replace the correctly
     // rounded sqrt, divide and powf; the error is a few ulp of fp32.
-    const f3 position = add3(add3(scale3(k[0], v0.p), scale3(k[1], v1.p)),
-                             scale3(k[2], v2.p));
+    // The pixel's eye position: the corners' eye positions interpolated
+    // (:280).  For a triangle in front of the near plane (not re-clipped,
+    // near test on) that is the point on the ray through the pixel centre
+    // at the interpolated eye depth (whose arithmetic is the reference's),
+    // found without the corners' eye x, y.
+    const f3 position =
+        eyes ? add3(add3(scale3(k[0], v0.p), scale3(k[1], v1.p)), scale3(k[2], v2.p))
+             : unproject_pixel(J, c, row, (k[0] * v0.p.z + k[1] * v1.p.z) + k[2] * v2.p.z);
     const f3 L = f3{J.light[0], J.light[1], J.light[2]};
     const f3 lightdir = J.light_is_point ? fast_norm3(sub3(L, position)) : L;
     f3 normal = add3(add3(scale3(k[0], v0.n), scale3(k[1], v1.n)), scale3(k[2], v2.n));
