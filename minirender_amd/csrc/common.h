@@ -70,11 +70,11 @@ struct JobDesc {
 };
 
 // Projected-vertex record written once per vertex by the vertex kernel:
-// eye-space position (Renderer.cpp:344-345) and its projection, in two
-// sections of one buffer: {px, py, zv, ez} (16 B, all that triangle setup
-// reads for an unclipped triangle) for every vertex, then {ex, ey} (8 B,
-// read by the near-plane clip and by shading).
-constexpr int kVrecFloats = 6;  // floats per vertex over both sections
+// {px, py, zv, ez}, the projection (Renderer.cpp:191-203) and eye depth
+// (:344-345) of the vertex, all that triangle setup reads for a triangle in
+// front of the near plane.  The eye x, y that the near-plane clip needs are
+// transformed again from the positions where it runs.
+constexpr int kVrecFloats = 4;
 
 // A surviving (sub-)triangle as the binner sees it: its id (2 * flat
 // triangle index + clip sub-index), its draw and clamped pixel bbox.

@@ -388,7 +388,6 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.block_stats = (unsigned long long*)s.block_stats.p;
   char* tb = (char*)s.tables.p;
   a.vscr = (float4*)s.vrec.p;
-  a.veye = (float2*)((char*)s.vrec.p + (size_t)p.total_verts * sizeof(float4));
   a.total_verts = p.total_verts;
   a.vg_begin = (const int64_t*)(tb + p.layout.vg_begin);
   a.vg_draw = (const int32_t*)(tb + p.layout.vg_draw);

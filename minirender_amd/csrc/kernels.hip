@@ -239,8 +239,8 @@ __device__ inline f3 load3(const float* p, int i) {
 
 // Corner indices of triangle t, their projections and eye depths
 // (Renderer.cpp:344-345, 364-366): one 16-byte load per corner.  The eye
-// x, y are left 0: corner_eyes loads them where the near-plane clip needs
-// them.
+// x, y are left 0: corner_eyes computes them where the near-plane clip
+// needs them.
 __device__ inline void corners(const BatchArgs& a, const DrawDesc& D, int t, int vi[3],
                                f3 e[3], ScreenVtx v[3]) {
   const auto* ip = G(D.idx) + 3 * t;
@@ -255,11 +255,17 @@ __device__ inline void corners(const BatchArgs& a, const DrawDesc& D, int t, int
   }
 }
 
-__device__ inline void corner_eyes(const BatchArgs& a, const DrawDesc& D, const int vi[3],
-                                   f3 e[3]) {
+// The corners' eye x, y, transformed again from the positions (the vertex
+// kernel's arithmetic on the same modelview: the same bits, Renderer.cpp:
+// 344-345); only a triangle across the near plane or of a draw without the
+// near test needs them.
+__device__ inline void corner_eyes(const DrawDesc& D, const int vi[3], f3 e[3]) {
+  f3 p[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) p[k] = load3(D.pos, vi[k]);
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    const float2 q = a.veye[D.vbase + vi[k]];
+    const f3 q = xform(D.mv, p[k]);
     e[k].x = q.x;
     e[k].y = q.y;
   }
@@ -284,7 +290,7 @@ __device__ inline int tri_setup(const BatchArgs& a, const DrawDesc& D, const Job
   if (!D.no_clip && (b0 || b1 || b2)) {
     if (b0 && b1 && b2) return 0;
     clipped = true;
-    corner_eyes(a, D, vi, e);
+    corner_eyes(D, vi, e);
     f3 a0, a1, a2, c0, c1, c2;
     const int ns = clip_positions(zn, e[0], e[1], e[2], a0, a1, a2, c0, c1, c2);
     int n = 0;
@@ -355,7 +361,7 @@ __device__ inline void tri_setup_sub(const BatchArgs& a, const DrawDesc& D, cons
   ScreenVtx v[3];
   corners(a, D, t, vi, e, v);
   if (!D.no_clip && (e[0].z > J.znear || e[1].z > J.znear || e[2].z > J.znear))
-    corner_eyes(a, D, vi, e);
+    corner_eyes(D, vi, e);
   tri_setup_corners(J, e, v, sub, D.no_clip != 0, s);
 }
 
@@ -373,7 +379,6 @@ __device__ inline void vertex_record(const BatchArgs& a, const DrawDesc& D, cons
   const f3 e = xform(D.mv, p);
   const ScreenVtx v = project_vertex(J.P, J.persp != 0, (float)J.W, (float)J.H, e);
   a.vscr[i] = make_float4(v.px, v.py, v.zv, e.z);
-  a.veye[i] = make_float2(e.x, e.y);
 }
 
 __device__ __forceinline__ void stage_vertex(const BatchArgs& a, int vb, int nvb) {
@@ -1276,7 +1281,7 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   }
   const bool clip = !D.no_clip && (ec[0].z > J.znear || ec[1].z > J.znear || ec[2].z > J.znear);
   const bool eyes = clip || D.no_clip;
-  if (eyes) corner_eyes(a, D, vi, ec);
+  if (eyes) corner_eyes(D, vi, ec);
   // Eye-space vertices as paintMesh computes them (Renderer.cpp:344-349).
   Vtx v0, v1, v2;
   v0.p = ec[0];

@@ -57,7 +57,6 @@ struct BatchArgs {
   // [kStatSlots, 2 kStatSlots) binned triangles per bin workgroup.
   unsigned long long* block_stats;
   float4* vscr;               // [total_verts] {px, py, zv, ez} of each projected vertex
-  float2* veye;               // [total_verts] {ex, ey} (after vscr in the same buffer)
   int64_t total_verts;
   const int64_t* vg_begin;    // [nvgroups] first record of each vertex group
   const int32_t* vg_draw;     // [nvgroups] a draw of the group (mesh, modelview, job)
