@@ -232,6 +232,17 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p) {
   return (__attribute__((address_space(1))) T*)p;
 }
 
+// A per-pixel output element at a 32-bit byte offset from its base: a frame
+// has at most kMaxDim^2 = 2^28 pixels, so offsets of up to 12 B per pixel
+// fit, and the access takes its (wavefront-uniform) base from scalar
+// registers instead of forming a 64-bit address per lane.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* px_at(T* base, uint32_t off) {
+  typedef __attribute__((address_space(1))) char GChar;
+  return (__attribute__((address_space(1))) T*)((GChar*)G(base) + off);
+}
+static_assert((uint64_t)kMaxDim * kMaxDim * 12 <= 0xFFFFFFFFull, "pixel offsets are 32-bit");
+
 __device__ inline f3 load3(const float* p, int i) {
   const auto* q = G(p);
   return f3{q[3 * i], q[3 * i + 1], q[3 * i + 2]};
@@ -1150,8 +1161,8 @@ __device__ __forceinline__ void stage_tile(const BatchArgs& a, int64_t b, bool r
       for (int p = tid + band_lo * TW; p < (band_hi + 1) * TW; p += NT) {
         const int c = x0 + (p % TW), row = y0 + (p / TW), lp = (p / TW) * KS + (p % TW);
         if (c < J.W && row < J.H) {
-          const size_t pix = (size_t)row * J.W + c;
-          G(J.win)[pix] = keys[lp] == ~0ull ? kNoWinner : (uint32_t)keys[lp];
+          const uint32_t pix = (uint32_t)row * (uint32_t)J.W + (uint32_t)c;
+          *px_at(J.win, 4u * pix) = keys[lp] == ~0ull ? kNoWinner : (uint32_t)keys[lp];
         }
       }
     } else {
@@ -1182,8 +1193,8 @@ __device__ __forceinline__ void stage_tile(const BatchArgs& a, int64_t b, bool r
             const int c = x0 + (p % TW), row = y0 + (p / TW);
             const u64 k = __hip_atomic_load(&gk[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (c < J.W && row < J.H) {
-              const size_t pix = (size_t)row * J.W + c;
-              G(J.win)[pix] = k == ~0ull ? kNoWinner : (uint32_t)k;
+              const uint32_t pix = (uint32_t)row * (uint32_t)J.W + (uint32_t)c;
+              *px_at(J.win, 4u * pix) = k == ~0ull ? kNoWinner : (uint32_t)k;
             }
           }
         }
@@ -1198,9 +1209,9 @@ __global__ __launch_bounds__(NT, 6) void k_tile(BatchArgs a) {
 }
 
 // Range image of the pixel from its depth (Renderer.cpp:388-415).
-__device__ inline void store_point(const JobDesc& J, size_t pix, int c, int row, float depth) {
+__device__ inline void store_point(const JobDesc& J, uint32_t pix, int c, int row, float depth) {
   const f3 q = range_point(J.P, range_far_depth(J.P), (float)J.W, (float)J.H, row, c, depth);
-  auto* o = G(J.points) + pix * 3;
+  auto* o = px_at(J.points, 12u * pix);
   o[0] = q.x;
   o[1] = q.y;
   o[2] = q.z;
@@ -1229,7 +1240,7 @@ __device__ inline f3 unproject_pixel(const JobDesc& J, int c, int row, float ze)
 
 // Shading of a covered pixel whose winner `id` belongs to draw D.
 __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, const DrawDesc& D,
-                                          uint32_t id, int c, int row, size_t pix) {
+                                          uint32_t id, int c, int row, uint32_t pix) {
   const int64_t g = id >> 1;
   const int sub = (int)(id & 1u);
   const int tri = (int)(g - D.tri_begin);
@@ -1317,8 +1328,8 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
   float k[3];
   const float z = depth_of(e1, e2, s.z0, s.z1, s.z2, J.persp != 0, k);
   if (!J.rgb8 && !J.rgb && !(J.normals && J.save_normals)) {
-    if (J.depth) G(J.depth)[pix] = z;
-    if (J.winner) G(J.winner)[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
+    if (J.depth) *px_at(J.depth, 4u * pix) = z;
+    if (J.winner) *px_at(J.winner, 4u * pix) = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
     if (J.points) store_point(J, pix, c, row, z);
     return;
   }
@@ -1387,52 +1398,52 @@ __device__ __forceinline__ void shade_lit(const BatchArgs& a, const JobDesc& J, 
       value = add3(value, scale3(sp, f3{D.specular[0], D.specular[1], D.specular[2]}));
     }
     if (save_n) {
-      auto* o = G(J.normals) + pix * 3;
+      auto* o = px_at(J.normals, 12u * pix);
       o[0] = normal.x;
       o[1] = normal.y;
       o[2] = normal.z;
     }
   } else if (save_n) {
-    auto* o = G(J.normals) + pix * 3;
+    auto* o = px_at(J.normals, 12u * pix);
     o[0] = 0.0f;
     o[1] = 0.0f;
     o[2] = 1.0f;
   }
   if (J.rgb8) {
-    auto* o = G(J.rgb8) + pix * 3;
+    auto* o = px_at(J.rgb8, 3u * pix);
     o[0] = quantize(value.x);
     o[1] = quantize(value.y);
     o[2] = quantize(value.z);
   }
   if (J.rgb) {
-    auto* o = G(J.rgb) + pix * 3;
+    auto* o = px_at(J.rgb, 12u * pix);
     o[0] = value.x;
     o[1] = value.y;
     o[2] = value.z;
   }
-  if (J.depth) G(J.depth)[pix] = z;
-  if (J.winner) G(J.winner)[pix] = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
+  if (J.depth) *px_at(J.depth, 4u * pix) = z;
+  if (J.winner) *px_at(J.winner, 4u * pix) = D.order_base + 2u * (uint32_t)tri + (uint32_t)sub;
   if (J.points) store_point(J, pix, c, row, z);
 }
 
 // A pixel nothing was drawn on: clear(), Renderer.cpp:113-119.
-__device__ inline void shade_background(const JobDesc& J, size_t pix, int c, int row) {
+__device__ inline void shade_background(const JobDesc& J, uint32_t pix, int c, int row) {
   if (J.rgb8) {
-    auto* o = G(J.rgb8) + pix * 3;
+    auto* o = px_at(J.rgb8, 3u * pix);
     o[0] = quantize(J.bg[0]);
     o[1] = quantize(J.bg[1]);
     o[2] = quantize(J.bg[2]);
   }
-  if (J.depth) G(J.depth)[pix] = kClearDepth;
-  if (J.winner) G(J.winner)[pix] = kNoWinner;
+  if (J.depth) *px_at(J.depth, 4u * pix) = kClearDepth;
+  if (J.winner) *px_at(J.winner, 4u * pix) = kNoWinner;
   if (J.rgb) {
-    auto* o = G(J.rgb) + pix * 3;
+    auto* o = px_at(J.rgb, 12u * pix);
     o[0] = J.bg[0];
     o[1] = J.bg[1];
     o[2] = J.bg[2];
   }
   if (J.normals && J.save_normals) {
-    auto* o = G(J.normals) + pix * 3;
+    auto* o = px_at(J.normals, 12u * pix);
     o[0] = 0.0f;
     o[1] = 0.0f;
     o[2] = 1.0f;
@@ -1444,7 +1455,7 @@ __device__ inline void shade_background(const JobDesc& J, size_t pix, int c, int
 __device__ void shade_pixel(const BatchArgs& a, const JobDesc& J,
                             const int64_t* dcache, int ncached, uint32_t id,
                             int c, int row) {
-  const size_t pix = (size_t)row * (size_t)J.W + (size_t)c;
+  const uint32_t pix = (uint32_t)row * (uint32_t)J.W + (uint32_t)c;
   if (id == kNoWinner) {
     shade_background(J, pix, c, row);
     return;
@@ -1534,12 +1545,12 @@ __device__ __forceinline__ void stage_shade(const BatchArgs& a, int vb, bool rem
   // draw-table fill below.  A tile without bin entries has no winners
   // written (the raster skips it): its pixels are background, and their
   // winners are not read.
-  const size_t pix = (size_t)row * J.W + c;
+  const uint32_t pix = (uint32_t)row * (uint32_t)J.W + (uint32_t)c;
   uint32_t id = kNoWinner;
   if (inside) {
     const int sw = __ffs(a.tile_w) - 1, sh = __ffs(a.tile_h) - 1;
     const uint32_t flag = a.tile_heavy[J.tile_base + (row >> sh) * J.tiles_x + (c >> sw)];
-    if (flag != kEmptyTile) id = G(J.win)[pix];
+    if (flag != kEmptyTile) id = *px_at(J.win, 4u * pix);
   }
   // A job of one draw needs no draw table (ncached = -1).
   const bool one_draw = J.draw_end - J.draw_begin == 1;
