@@ -948,7 +948,7 @@ __device__ __forceinline__ void stage_bin(const BatchArgs& a, int vb, int nvb) {
   if (threadIdx.x == 0 && vb < kStatSlots) a.block_stats[kStatSlots + vb] = sh_items;
 }
 
-__global__ __launch_bounds__(256, 8) void k_bin(BatchArgs a) {
+__global__ __launch_bounds__(256, 7) void k_bin(BatchArgs a) {
   stage_bin(a, blockIdx.x, gridDim.x);
 }
 

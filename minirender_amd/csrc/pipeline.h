@@ -106,7 +106,7 @@ __host__ __device__ inline int32_t geom_grid(const BatchArgs& a) {
 __host__ __device__ inline int32_t bin_grid(const BatchArgs& a) {
   int64_t blocks = (a.nregions + 3) / 4;
   if (blocks < 1) blocks = 1;
-  const int64_t cap = (int64_t)a.num_cus * 8;
+  const int64_t cap = (int64_t)a.num_cus * 7;  // one resident round (7 per CU)
   return (int32_t)(blocks < cap ? blocks : cap);
 }
 
