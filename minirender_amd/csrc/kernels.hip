@@ -1004,11 +1004,11 @@ __global__ __launch_bounds__(256) void k_rows(BatchArgs a) {
 }
 
 // Depth test of one pixel into the tile's LDS key buffer.
-__device__ inline void raster_pixel(u64* keys, int p, float e1, float e2,
-                                    const TriSetup& s, bool persp, uint32_t id) {
+__device__ inline void raster_pixel(u64* keys, int p, float e1, float e2, float z0, float z1,
+                                    float z2, bool persp, uint32_t id) {
   if (!covered(e1, e2)) return;
   float k[3];
-  const float z = depth_of(e1, e2, s.z0, s.z1, s.z2, persp, k);
+  const float z = depth_of(e1, e2, z0, z1, z2, persp, k);
   if (!(z < kClearDepth)) return;  // strict test against the clear value
   const u64 key = ((u64)depth_key(z) << 32) | id;
   if (key < keys[p]) atomicMin(&keys[p], key);
@@ -1147,11 +1147,14 @@ __device__ __forceinline__ void stage_tile(const BatchArgs& a, int64_t b, bool r
                      (float)s.bx0 + 0.5f, (float)row + 0.5f, e1, e2);
           advance2(e1, s.n1x, e2, s.n2x, cx0 - s.bx0);
         }
+        // The triangle's steps and depths in registers for the row: read from
+        // the workgroup's setup table once, not again after each key update.
+        const float n1x = s.n1x, n2x = s.n2x, z0 = s.z0, z1 = s.z1, z2 = s.z2;
         const int prow = (row - y0) * KS - x0;
         for (int c = cx0; c <= cx1; c++) {
-          raster_pixel(keys, prow + c, e1, e2, s, persp, id);
-          e1 = e1 + s.n1x;
-          e2 = e2 + s.n2x;
+          raster_pixel(keys, prow + c, e1, e2, z0, z1, z2, persp, id);
+          e1 = e1 + n1x;
+          e2 = e2 + n2x;
         }
       }
       __syncthreads();
