@@ -1159,13 +1159,18 @@ __device__ __forceinline__ void stage_tile(const BatchArgs& a, int64_t b, bool r
       }
       __syncthreads();
     }
+    // The job's size and winner buffer in registers: the stores below could
+    // alias the job table for all the compiler knows, which would reload
+    // them for every pixel.
+    const int jw = J.W, jh = J.H;
+    uint32_t* const jwin = J.win;
     if (nparts == 1) {
       // (a band writes its own rows only)
       for (int p = tid + band_lo * TW; p < (band_hi + 1) * TW; p += NT) {
         const int c = x0 + (p % TW), row = y0 + (p / TW), lp = (p / TW) * KS + (p % TW);
-        if (c < J.W && row < J.H) {
-          const uint32_t pix = (uint32_t)row * (uint32_t)J.W + (uint32_t)c;
-          *px_at(J.win, 4u * pix) = keys[lp] == ~0ull ? kNoWinner : (uint32_t)keys[lp];
+        if (c < jw && row < jh) {
+          const uint32_t pix = (uint32_t)row * (uint32_t)jw + (uint32_t)c;
+          *px_at(jwin, 4u * pix) = keys[lp] == ~0ull ? kNoWinner : (uint32_t)keys[lp];
         }
       }
     } else {
@@ -1195,9 +1200,9 @@ __device__ __forceinline__ void stage_tile(const BatchArgs& a, int64_t b, bool r
           for (int p = tid; p < NP; p += NT) {
             const int c = x0 + (p % TW), row = y0 + (p / TW);
             const u64 k = __hip_atomic_load(&gk[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (c < J.W && row < J.H) {
-              const uint32_t pix = (uint32_t)row * (uint32_t)J.W + (uint32_t)c;
-              *px_at(J.win, 4u * pix) = k == ~0ull ? kNoWinner : (uint32_t)k;
+            if (c < jw && row < jh) {
+              const uint32_t pix = (uint32_t)row * (uint32_t)jw + (uint32_t)c;
+              *px_at(jwin, 4u * pix) = k == ~0ull ? kNoWinner : (uint32_t)k;
             }
           }
         }
