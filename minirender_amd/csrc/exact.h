@@ -32,6 +32,14 @@ MR_HD inline double pow2d(int e) {  // exact 2^e for -1022 <= e <= 1023
 MR_HD inline float amin(float a, float b) { return a < b ? a : b; }
 MR_HD inline float amax(float a, float b) { return a > b ? a : b; }
 MR_HD inline float aclamp(float x, float a, float b) { return amin(amax(x, a), b); }
+// The bounding-box minimum / maximum of triangle setup, as the hardware
+// min / max (one instruction; amin and amax compile to a compare and a
+// select).  Equal to amin / amax but for NaN and the sign of a zero: a zero
+// of either sign clamps and floors to the same pixel, and a triangle with a
+// NaN coordinate has a NaN area and is rejected by setup either way.
+MR_HD inline float bmin(float a, float b) { return fminf(a, b); }
+MR_HD inline float bmax(float a, float b) { return fmaxf(a, b); }
+MR_HD inline float bclamp(float x, float a, float b) { return bmin(bmax(x, a), b); }
 
 MR_HD inline f3 add3(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
 MR_HD inline f3 sub3(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
@@ -360,10 +368,10 @@ MR_HD inline int last_center(float hi) {
 MR_HD inline bool setup_screen(float w, float h, const ScreenVtx& v0, const ScreenVtx& v1,
                                const ScreenVtx& v2, TriSetup& r) {
   const float p0x = v0.px, p0y = v0.py, p1x = v1.px, p1y = v1.py, p2x = v2.px, p2y = v2.py;
-  float minx = amin(amin(amin(1e30f, p0x), p1x), p2x);
-  float miny = amin(amin(amin(1e30f, p0y), p1y), p2y);
-  float maxx = amax(amax(amax(-1e30f, p0x), p1x), p2x);
-  float maxy = amax(amax(amax(-1e30f, p0y), p1y), p2y);
+  float minx = bmin(bmin(bmin(1e30f, p0x), p1x), p2x);
+  float miny = bmin(bmin(bmin(1e30f, p0y), p1y), p2y);
+  float maxx = bmax(bmax(bmax(-1e30f, p0x), p1x), p2x);
+  float maxy = bmax(bmax(bmax(-1e30f, p0y), p1y), p2y);
   if (maxx < 0 || maxy < 0 || minx > w || miny > h) return false;
   float ar = (p0x - p1x) * (p2y - p1y) - (p0y - p1y) * (p2x - p1x);
   if (!(ar > 0.0f)) {
@@ -377,10 +385,10 @@ MR_HD inline bool setup_screen(float w, float h, const ScreenVtx& v0, const Scre
   r.n2x = -(p1y - p0y) * i2a;
   r.n2y = (p1x - p0x) * i2a;
   r.p0x = p0x; r.p0y = p0y; r.p2x = p2x; r.p2y = p2y;
-  minx = aclamp(minx, 0.0f, w - 1);
-  maxx = aclamp(maxx, 0.0f, w - 1);
-  miny = aclamp(miny, 0.0f, h - 1);
-  maxy = aclamp(maxy, 0.0f, h - 1);
+  minx = bclamp(minx, 0.0f, w - 1);
+  maxx = bclamp(maxx, 0.0f, w - 1);
+  miny = bclamp(miny, 0.0f, h - 1);
+  maxy = bclamp(maxy, 0.0f, h - 1);
   r.bx0 = (int)floorf(minx);
   r.bx1 = last_center(maxx);
   r.by0 = (int)floorf(miny);
@@ -397,8 +405,8 @@ MR_HD inline bool setup_screen(float w, float h, const ScreenVtx& v0, const Scre
 MR_HD inline void setup_edges(float w, const ScreenVtx& v0, const ScreenVtx& v1,
                               const ScreenVtx& v2, TriSetup& r) {
   const float p0x = v0.px, p0y = v0.py, p1x = v1.px, p1y = v1.py, p2x = v2.px, p2y = v2.py;
-  const float minx = amin(amin(amin(1e30f, p0x), p1x), p2x);
-  const float maxx = amax(amax(amax(-1e30f, p0x), p1x), p2x);
+  const float minx = bmin(bmin(bmin(1e30f, p0x), p1x), p2x);
+  const float maxx = bmax(bmax(bmax(-1e30f, p0x), p1x), p2x);
   const float ar = (p0x - p1x) * (p2y - p1y) - (p0y - p1y) * (p2x - p1x);
   const float i2a = -1.0f / ar;
   r.n1x = -(p0y - p2y) * i2a;
@@ -406,8 +414,8 @@ MR_HD inline void setup_edges(float w, const ScreenVtx& v0, const ScreenVtx& v1,
   r.n2x = -(p1y - p0y) * i2a;
   r.n2y = (p1x - p0x) * i2a;
   r.p0x = p0x; r.p0y = p0y; r.p2x = p2x; r.p2y = p2y;
-  r.bx0 = (int)floorf(aclamp(minx, 0.0f, w - 1));
-  r.bx1 = last_center(aclamp(maxx, 0.0f, w - 1));
+  r.bx0 = (int)floorf(bclamp(minx, 0.0f, w - 1));
+  r.bx1 = last_center(bclamp(maxx, 0.0f, w - 1));
   r.z0 = v0.zv;
   r.z1 = v1.zv;
   r.z2 = v2.zv;
