@@ -50,8 +50,19 @@ sys.path.insert(0, ROOT)
 METRIC = "Mtris/s + frames/s at 1080p; p50/p99 ms/request, batched vs plain GPU"
 HBM_PEAK_GBS = 8000.0
 STAGES = ["clear", "geometry", "scan", "bin", "rows", "raster", "shade"]
-KERNEL_OF = {"clear": "k_clear", "geometry": "k_vertex+k_geom", "scan": "k_scan", "bin": "k_bin",
-             "rows": "k_rows", "raster": "k_tile", "shade": "k_shade"}
+CKPT_SCAN_REGIONS = 65536  # batches of at most this many 64-triangle regions: k_bin_rows
+
+
+def kernel_of(frame):
+    """The kernel behind each stage of a frame rendered alone.  A batch of at
+    most CKPT_SCAN_REGIONS regions takes its edge checkpoints in the scan's
+    launch and walks their rows at the end of binning (k_bin_rows, no k_rows
+    launch); a larger one bins with k_bin and walks with k_rows."""
+    small = (frame.ntri + 63) // 64 <= CKPT_SCAN_REGIONS
+    return {"clear": "k_clear", "geometry": "k_vertex+k_geom", "scan": "k_scan",
+            "bin": "k_bin_rows" if small else "k_bin",
+            "rows": "(in k_bin_rows)" if small else "k_rows", "raster": "k_tile",
+            "shade": "k_shade"}
 
 
 def parse():
@@ -249,11 +260,12 @@ def plain_section(args, torch, mra, frame, wl_name):
     ctx.set_profiling(False)
     stage /= args.steps
     alg = kernel_alg_bytes(frame)
+    kname = kernel_of(frame)
     kernels = {}
     for i, name in enumerate(STAGES):
         ms = float(stage[i])
         gbs = alg[name] / (ms * 1e-3) / 1e9 if ms > 0 and alg[name] else 0.0
-        kernels[KERNEL_OF[name]] = {"ms": round(ms, 5), "alg_bytes": alg[name],
+        kernels[kname[name]] = {"ms": round(ms, 5), "alg_bytes": alg[name],
                                     "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     dom = max(range(1, len(STAGES)), key=lambda i: stage[i])
     dname = STAGES[dom]
@@ -267,19 +279,22 @@ def plain_section(args, torch, mra, frame, wl_name):
     # so it is reported beside the headline with its counter traffic.
     ktraffic = {}
     for n in STAGES[1:]:
-        parts = [read_traffic(k, wl_name) for k in KERNEL_OF[n].split("+")]
-        ktraffic[KERNEL_OF[n]] = sum(parts) if all(v is not None for v in parts) else None
+        # (a stage without a launch of its own moves nothing)
+        parts = [0.0 if k.startswith("(") else read_traffic(k, wl_name)
+                 for k in kname[n].split("+")]
+        ktraffic[kname[n]] = sum(parts) if all(v is not None for v in parts) else None
     frame_traffic = (sum(ktraffic.values()) if all(v is not None for v in ktraffic.values())
                      else None)
     achieved = fb / (fmed * 1e-3) / 1e9 if fmed > 0 else 0.0
-    roof = {"bound": "hbm", "kernel": "frame (k_vertex, k_geom, k_scan, k_bin, k_rows, k_tile, k_shade)",
+    launched = [k for n in STAGES[1:] for k in kname[n].split("+") if not k.startswith("(")]
+    roof = {"bound": "hbm", "kernel": "frame (" + ", ".join(launched) + ")",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "alg_bytes_per_launch": fb, "avg_launch_ms": round(fmed, 5),
             "traffic": frame_traffic,
             "traffic_over_alg": round(frame_traffic / fb, 2) if frame_traffic else None}
-    dk = {"kernel": KERNEL_OF[dname], "avg_launch_ms": round(dms, 5),
-          "alg_bytes_per_launch": alg[dname], "traffic": ktraffic.get(KERNEL_OF[dname])}
+    dk = {"kernel": kname[dname], "avg_launch_ms": round(dms, 5),
+          "alg_bytes_per_launch": alg[dname], "traffic": ktraffic.get(kname[dname])}
     # How hard the dominant kernel drives HBM: its measured traffic (from the
     # committed counter summary) over its launch time.
     if dk["traffic"] and dms > 0:
@@ -288,7 +303,7 @@ def plain_section(args, torch, mra, frame, wl_name):
         dk["traffic_frac"] = round(tgbs / HBM_PEAK_GBS, 4)
     roof["dominant"] = dk
     for name in STAGES[1:]:
-        kernels[KERNEL_OF[name]]["traffic"] = ktraffic.get(KERNEL_OF[name])
+        kernels[kname[name]]["traffic"] = ktraffic.get(kname[name])
     res = {"ms_per_step": dt * 1e3, "value": frame.ntri / dt / 1e6, "frames_per_s": 1.0 / dt,
            "tile": tile, "autotune": tune, "kernels": kernels, "roofline": roof,
            "records": st.records, "bin_entries": st.bin_entries,

@@ -351,6 +351,7 @@ static BatchArgs make_args(mr_ctx* c, Slot& sl, const BatchPlan& p) {
   a.items = (BinItem*)s.items.p;
   a.region_count = (uint32_t*)s.region_count.p;
   a.nregions = (p.total_tris + 63) / 64;
+  a.ckpt_in_scan = a.nregions <= kCkptScanRegions ? 1 : 0;
   a.tile_count = (unsigned long long*)s.tile_count.p;
   a.tile_off = (uint32_t*)s.tile_off.p;
   a.tile_cursor = (uint32_t*)s.tile_cursor.p;
@@ -430,7 +431,8 @@ static int launch_stages(const BatchArgs& a, Slot& s, bool profile, bool clear,
     s.last_launches = 1;
     return MR_OK;
   }
-  s.last_launches = (clear ? 1 : 0) + (a.total_verts ? 1 : 0) + (a.total_tris ? 1 : 0) + 5;
+  s.last_launches = (clear ? 1 : 0) + (a.total_verts ? 1 : 0) + (a.total_tris ? 1 : 0) +
+                    (a.ckpt_in_scan ? 4 : 5);
   if (profile) HIP_TRY(hipEventRecord(s.ev[0], st));
   if (clear) HIP_TRY(launch_clear(a, st));
   if (profile) HIP_TRY(hipEventRecord(s.ev[1], st));
@@ -541,8 +543,9 @@ int enqueue_batch(mr_ctx* c, Slot& s, BatchPlan& p, bool profile) {
     // geometry launches, raster template, persistent or not) updates the
     // parameters of an instantiated graph in place instead of instantiating
     // a new one, so a service of mixed batches keeps replaying a few graphs.
-    const int32_t topo_v[6] = {clear ? 1 : 0, a.total_verts ? 1 : 0, a.total_tris ? 1 : 0,
-                               a.tile_w, a.raster_threads, persistent ? 1 : 0};
+    const int32_t topo_v[7] = {clear ? 1 : 0, a.total_verts ? 1 : 0, a.total_tris ? 1 : 0,
+                               a.tile_w, a.raster_threads, persistent ? 1 : 0,
+                               persistent ? 0 : a.ckpt_in_scan};
     std::vector<unsigned char> topo((const unsigned char*)topo_v,
                                     (const unsigned char*)topo_v + sizeof(topo_v));
     if (hit == s.graphs.size()) {

@@ -1,8 +1,10 @@
 // kernels.hip -- the rasterization pipeline of minirender's Renderer::render()
 // (src/Renderer.cpp:311-381) as HIP kernels for gfx950 (wave64).
 //
-//   k_clear   look-back words, tile counts and merge counters, when the
-//             scratch is new (afterwards the shading kernel leaves them zero)
+//   k_clear   look-back words, tile counts and merge counters when the
+//             scratch is new (afterwards the shading kernel leaves them
+//             zero); batch and checkpoint counters when the batch has no
+//             triangles (else geometry zeroes them)
 //   k_vertex  (a) eye-space transform and projection once per vertex of each
 //             (attribute arrays, modelview) group of a job (Renderer.cpp:340-349,
 //             191-203): the per-material meshes of a model share one group
@@ -13,13 +15,19 @@
 //             (tile, triangle) pairs with one atomic per distinct tile
 //   k_scan    (c) exclusive scan of the tile counts (decoupled look-back);
 //             tiles with more pairs than one work item are split into parts,
-//             the others with pairs listed for the raster, empty ones marked
+//             the others with pairs listed for the raster, empty ones marked.
+//             For a batch of at most kCkptScanRegions regions, the launch's
+//             other workgroups, beside the one-workgroup scan, allocate the
+//             edge checkpoints and row tasks of triangles at least 64 pixels
+//             wide spanning several tile columns
 //   k_bin     (c) scatter of (triangle id, draw) into the tiles' bin lists:
 //             per wavefront step, one atomic instruction for all distinct
-//             tiles; allocates the edge checkpoints and row tasks of
-//             triangles at least 64 pixels wide spanning several tile columns
+//             tiles.  For larger batches binning allocates the checkpoints
+//             itself (it reads the items anyway)
 //   k_rows    (c) row pre-pass: one lane per row of such a triangle walks it
-//             once and stores the edge values where it enters each tile column
+//             once and stores the edge values where it enters each tile
+//             column; for the smaller batches the tail of the binning
+//             kernel (k_bin_rows)
 //   k_tile    (d) per-tile raster (listed tiles, parts of split tiles, row
 //             bands) with the tile's 64-bit (depth key, draw-order id) buffer
 //             in LDS and ds_min_u64; parts of a split tile merge through
@@ -94,6 +102,10 @@ __device__ u64 lookback(u64* status, int idx, u64 aggregate, u64* counters) {
 // nvb workgroups of 256 threads: run by its own kernel (vb = blockIdx.x),
 // or by the persistent kernel, whose workgroups take stage work from queues.
 __device__ __forceinline__ void stage_clear(const BatchArgs& a, int vb, int nvb) {
+  // The checkpoint segment counters too: a batch without triangles (no
+  // geometry, which zeroes them otherwise) must not leave the row pre-pass
+  // the last batch's row tasks.
+  if (vb == 0 && threadIdx.x < kCkptSegs) a.ckpt_ctr[threadIdx.x * kCtrStride] = 0;
   const int n = max(max(a.scan_blocks, a.ntiles), (int)kNumCounters);
   for (int i = vb * 256 + (int)threadIdx.x; i < n; i += nvb * 256) {
     if (i < kNumCounters) a.counters[i] = 0;
@@ -798,7 +810,6 @@ __device__ __forceinline__ void stage_scan(const BatchArgs& a) {
   }
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) { stage_scan(a); }
 
 // Edge checkpoints of the wavefront's items: an item at least
 // kCkptMinSpan pixels wide that spans ntx > 1 tile columns gets ntx entries
@@ -881,8 +892,10 @@ __device__ inline void ckpt_alloc(const BatchArgs& a, bool act, const BinItem& i
   }
 }
 
-// (c) scatter triangle ids into the bin lists of the tiles they overlap and
-// take their edge checkpoints; also resets the merge buffers of split tiles.
+// (c) scatter triangle ids into the bin lists of the tiles they overlap;
+// also resets the merge buffers of split tiles.  kAlloc: also take the
+// edge checkpoints of the items (else the scan's launch has taken them).
+template <bool kAlloc>
 __device__ __forceinline__ void stage_bin(const BatchArgs& a, int vb, int nvb) {
   const int lane = threadIdx.x & 63;
   const u64 np = (u64)a.tile_w * a.tile_h;
@@ -932,7 +945,7 @@ __device__ __forceinline__ void stage_bin(const BatchArgs& a, int vb, int nvb) {
         tbase = a.draws[dd].tile_base;
         tiles_x = a.draws[dd].tiles_x;
       }
-      ckpt_alloc(a, act, it, sw, (uint32_t)(r * kRegion + j + lane), vb);
+      if (kAlloc) ckpt_alloc(a, act, it, sw, (uint32_t)(r * kRegion + j + lane), vb);
       wave_tile_pairs<true>(a, act, it.id, it.draw, tbase, tiles_x, it.bx0, it.bx1, it.by0,
                             it.by1, sw, sh, reserve);
     }
@@ -948,9 +961,6 @@ __device__ __forceinline__ void stage_bin(const BatchArgs& a, int vb, int nvb) {
   if (threadIdx.x == 0 && vb < kStatSlots) a.block_stats[kStatSlots + vb] = sh_items;
 }
 
-__global__ __launch_bounds__(256, 7) void k_bin(BatchArgs a) {
-  stage_bin(a, blockIdx.x, gridDim.x);
-}
 
 // Row pre-pass: one lane per row task of an item with edge checkpoints (a
 // triangle at least kCkptMinSpan pixels wide spanning several tile
@@ -997,6 +1007,49 @@ __device__ __forceinline__ void stage_rows(const BatchArgs& a, int vb, int nvb) 
       out[tx] = make_float2(e1, e2);
     }
   }
+}
+
+// The edge checkpoints of every item (allocation and row tasks), over the
+// regions geometry wrote (wavefront w of the grid takes regions w, w + 4 nvb,
+// ...).
+__device__ __forceinline__ void stage_ckpt(const BatchArgs& a, int vb, int nvb) {
+  const int lane = threadIdx.x & 63;
+  const int sw = __ffs(a.tile_w) - 1;
+  const int64_t nw = (int64_t)nvb * 4;
+  int64_t r = (int64_t)vb * 4 + (threadIdx.x >> 6);
+  uint32_t cnt = r < a.nregions ? a.region_count[r] : 0u;
+  while (r < a.nregions) {
+    const int64_t rn = r + nw;
+    const uint32_t cnt_n = rn < a.nregions ? a.region_count[rn] : 0u;
+    for (uint32_t j = 0; j < cnt; j += 64) {
+      const bool act = j + lane < cnt;
+      const BinItem it = act ? a.items[r * kRegion + j + lane] : BinItem{0, 0, 0, 0, 0, 0};
+      ckpt_alloc(a, act, it, sw, (uint32_t)(r * kRegion + j + lane), vb);
+    }
+    r = rn;
+    cnt = cnt_n;
+  }
+}
+
+// The scan's workgroups first, then (ckpt_in_scan) the checkpoint
+// allocation's, which needs only geometry's items and so runs beside the
+// scan: one launch for both.
+__global__ __launch_bounds__(kScanBlock) void k_scan(BatchArgs a) {
+  static_assert(kScanBlock == 256, "the checkpoint pass runs in workgroups of 256");
+  if ((int)blockIdx.x < a.scan_blocks) stage_scan(a);
+  else stage_ckpt(a, (int)blockIdx.x - a.scan_blocks, (int)gridDim.x - a.scan_blocks);
+}
+
+// Binning that takes the checkpoints (k_rows walks them next), and binning
+// followed by the row pre-pass of the row tasks the scan's launch wrote
+// (ckpt_in_scan): kernels of their own, each with only its own code.
+__global__ __launch_bounds__(256, 7) void k_bin(BatchArgs a) {
+  stage_bin<true>(a, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(256, 7) void k_bin_rows(BatchArgs a) {
+  stage_bin<false>(a, blockIdx.x, gridDim.x);
+  stage_rows(a, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(256) void k_rows(BatchArgs a) {
@@ -1672,7 +1725,7 @@ __device__ __forceinline__ void persist_run(const BatchArgs& a, const PersistPla
     case kPVertex: stage_vertex(a, (int)vb, q.nvb[s]); break;
     case kPGeom: stage_geom(a, (int)vb, q.nvb[s]); break;
     case kPScan: stage_scan(a); break;
-    case kPBin: stage_bin(a, (int)vb, q.nvb[s]); break;
+    case kPBin: stage_bin<true>(a, (int)vb, q.nvb[s]); break;
     case kPRows: stage_rows(a, (int)vb, q.nvb[s]); break;
     case kPTile: stage_tile<TW, TW, 256>(a, vb, false); break;
     default: stage_shade(a, (int)vb, false); break;
@@ -1917,19 +1970,25 @@ hipError_t launch_geometry(const BatchArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_scan(const BatchArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_scan, dim3(a.scan_blocks), dim3(kScanBlock), 0, s, a);
+  const int32_t ckpt_blocks = a.ckpt_in_scan ? bin_grid(a) : 0;
+  hipLaunchKernelGGL(k_scan, dim3(a.scan_blocks + ckpt_blocks), dim3(kScanBlock), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_bin(const BatchArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin, dim3(bin_grid(a)), dim3(256), 0, s, a);
+  if (a.ckpt_in_scan)
+    hipLaunchKernelGGL(k_bin_rows, dim3(bin_grid(a)), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_bin, dim3(bin_grid(a)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_rows(const BatchArgs& a, hipStream_t s) {
+  if (a.ckpt_in_scan) return hipSuccess;
   hipLaunchKernelGGL(k_rows, dim3(bin_grid(a)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
+
 
 bool tile_supported(int tw, int th) {
   return (tw == 8 && th == 8) || (tw == 16 && th == 16) || (tw == 32 && th == 32);

@@ -80,7 +80,16 @@ struct BatchArgs {
   int32_t shade_blocks;
   int32_t num_cus;
   int32_t geom_blocks;       // resident geometry workgroups (<= kStatSlots)
+  // Where the edge checkpoints are taken (launched stages; the persistent
+  // kernel always bins first): 1, by the scan's launch in workgroups beside
+  // the scan, with the row pre-pass at the end of the binning kernel (a
+  // batch of at most kCkptScanRegions regions, whose items that pass reads
+  // while the one-workgroup scan runs); 0, by the binning kernel, which reads
+  // the items anyway, with the row pre-pass a launch of its own.
+  int32_t ckpt_in_scan;
 };
+
+constexpr int64_t kCkptScanRegions = 65536;
 
 constexpr int kStatSlots = 8192;  // >= the geometry and bin grids
 
@@ -115,7 +124,7 @@ hipError_t launch_vertex(const BatchArgs& a, hipStream_t s);
 hipError_t launch_geometry(const BatchArgs& a, hipStream_t s);
 hipError_t launch_scan(const BatchArgs& a, hipStream_t s);
 hipError_t launch_bin(const BatchArgs& a, hipStream_t s);
-hipError_t launch_rows(const BatchArgs& a, hipStream_t s);
+hipError_t launch_rows(const BatchArgs& a, hipStream_t s);  // nothing when ckpt_in_scan
 hipError_t launch_tile(const BatchArgs& a, hipStream_t s);
 // Geometry workgroups one CU holds at once (from the kernel's resources).
 int geom_blocks_per_cu();

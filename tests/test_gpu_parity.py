@@ -195,6 +195,23 @@ def test_empty_and_degenerate(gpu_ctx):
     check(gpu_ctx, fr, "degenerate")
 
 
+def test_empty_frame_after_wide_triangles(gpu_ctx):
+    """A frame without triangles right after one whose wide triangles get
+    edge checkpoints, on the same context, then the wide frame again: the
+    empty frame is background (its row pre-pass finds no row tasks of the
+    frame before) and the repeat equals the first render bit for bit."""
+    kinds = ("rgb8", "depth", "winner")
+    tris = scenes.random_triangles(12, 9, lo=2.0, hi=6.0, center_lo=(-1, -1, -3), center_hi=(1, 1, -2))
+    wide = FrameSpec(640, 360, scenes.projection_frustum(math.radians(60), 16 / 9, 0.1, 50.0),
+                     draws=[Draw(tris, Material(shininess=20.0))], light=(0.3, 0.5, 1.0))
+    g1, _ = check(gpu_ctx, wide, "wide", kinds)
+    empty = FrameSpec(640, 360, wide.projection, background=(0.2, 0.4, 0.6))
+    check(gpu_ctx, empty, "empty after wide", kinds)
+    g2 = gpu_ctx.render(wide, kinds)
+    for k in kinds:
+        assert np.array_equal(g1[k], g2[k]), k
+
+
 def test_shared_edges_first_wins(gpu_ctx):
     """A pixel-aligned quad split along its diagonal, drawn twice with
     different materials: ties go to the first-drawn triangle (strict z test)
