@@ -236,8 +236,8 @@ def plain_section(args, torch, mra, frame, wl_name):
     log(f"plain: {tune}")
     # Timed steps: synchronous mr_render calls with eager launches (on this
     # GPU and ROCm, replaying the frame as a captured HIP graph costs more
-    # host time per frame than the seven launches it replaces), no per-stage
-    # events.
+    # host time per frame than the six or seven launches it replaces), no
+    # per-stage events.
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
