@@ -81,11 +81,12 @@ struct BatchArgs {
   int32_t num_cus;
   int32_t geom_blocks;       // resident geometry workgroups (<= kStatSlots)
   // Where the edge checkpoints are taken (launched stages; the persistent
-  // kernel always bins first): 1, by the scan's launch in workgroups beside
-  // the scan, with the row pre-pass at the end of the binning kernel (a
-  // batch of at most kCkptScanRegions regions, whose items that pass reads
-  // while the one-workgroup scan runs); 0, by the binning kernel, which reads
-  // the items anyway, with the row pre-pass a launch of its own.
+  // kernel always takes them in its binning stage): 1, by the scan's launch
+  // in workgroups beside the scan, with the row pre-pass at the end of the
+  // binning kernel (a batch of at most kCkptScanRegions regions, whose items
+  // that pass reads while the one-workgroup scan runs); 0, by the binning
+  // kernel, which reads the items anyway, with the row pre-pass a launch of
+  // its own.
   int32_t ckpt_in_scan;
 };
 
